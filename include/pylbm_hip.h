@@ -1,0 +1,200 @@
+/*
+ * pylbm_hip.h -- C ABI of the MI355X (gfx950) lattice Boltzmann time-stepper.
+ *
+ * Two kinds of shared objects export these symbols:
+ *
+ *   libpylbm_hip.so       core runtime: device memory, copies, streams, events,
+ *                         and the scheme-independent kernels (periodic ghost
+ *                         fill, boundary link lists, FP64 compensated sums).
+ *   lbm_gen_<hash>.so     one per scheme, generated from the scheme's moment
+ *                         matrix / equilibrium / relaxation rates: the fused
+ *                         pull-stream + collide kernel and the standalone
+ *                         f2m / m2f / equilibrium kernels, plus a multi-step
+ *                         driver.
+ *
+ * Every entry point returns an int status: 0 = success, > 0 = a hipError_t,
+ * < 0 = an argument error; lbm_last_error() returns the message of the last
+ * failure on the calling thread.  Pointers named d_* are device pointers,
+ * h_* host pointers.  Streams are hipStream_t passed as void* (NULL = the
+ * default stream).  No per-step entry point allocates device memory except the
+ * link driver when a boundary method needs its two-phase form (it uses a
+ * workspace the caller passes in, see lbm_links_apply).
+ *
+ * The reference routines each entry point replaces (pylbm 0.11.0) are cited
+ * as file:line; the Python module pylbm_amd binds these names with ctypes and
+ * exposes them under pylbm's routine names (transport / f2m / m2f /
+ * equilibrium / one_time_step / bounce_back / anti_bounce_back / neumann*).
+ */
+#ifndef PYLBM_HIP_H
+#define PYLBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* population storage formats (the compute type is FP64 for LBM_F64, FP32 otherwise) */
+enum lbm_storage {
+  LBM_F64 = 0,   /* double                                                   */
+  LBM_F32 = 1,   /* float                                                    */
+  LBM_F16D = 2,  /* IEEE half of the deviation f_k - w_k rho0               */
+  LBM_BF16D = 3  /* bfloat16 of the deviation f_k - w_k rho0                */
+};
+
+#define LBM_MAX_Q 32
+
+/*
+ * Device layout of one population array (structure of arrays).
+ * Spatial axes are canonicalised to (outer, mid, inner): 3D (x, y, z),
+ * 2D (1, x, y), 1D (1, 1, x) -- the last reference axis is the contiguous one,
+ * as in pylbm's SOA storage (pylbm/storage.py:423-472).  Extents include the
+ * halo of width halo[a].  Element (k, o, m, i) lives at
+ *   k*pop_stride + (o*n[1] + m)*pitch + lead + i
+ * with pitch a multiple of 128 bytes and lead chosen so that the first
+ * interior element of every row is 128-byte aligned.
+ */
+typedef struct lbm_layout {
+  int32_t dim;
+  int32_t ns;
+  int32_t storage;     /* enum lbm_storage */
+  int32_t elem_size;   /* bytes per stored population value */
+  int64_t n[3];        /* halo-inclusive extents (outer, mid, inner) */
+  int32_t halo[3];
+  int32_t _pad;
+  int64_t pitch;       /* elements per inner row (padded) */
+  int64_t lead;        /* leading pad elements of each row */
+  int64_t pop_stride;  /* elements between two populations */
+} lbm_layout;
+
+/* canonical velocity table shared by every kernel of one scheme */
+typedef struct lbm_velocities {
+  int32_t q;
+  int32_t _pad;
+  int8_t v[LBM_MAX_Q][3];  /* canonical (outer, mid, inner) components */
+} lbm_velocities;
+
+/* ------------------------------------------------------------------ core */
+const char* lbm_last_error(void);
+int lbm_device_count(int* count);
+int lbm_set_device(int device);
+int lbm_device_name(int device, char* buf, int len);
+int lbm_malloc(void** d_ptr, uint64_t bytes);              /* 256-byte aligned */
+int lbm_free(void* d_ptr);
+int lbm_memcpy_h2d(void* d_dst, const void* h_src, uint64_t bytes, void* stream);
+int lbm_memcpy_d2h(void* h_dst, const void* d_src, uint64_t bytes, void* stream);
+int lbm_memcpy_d2d(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
+int lbm_memset(void* d_ptr, int value, uint64_t bytes, void* stream);
+int lbm_stream_create(void** stream);
+int lbm_stream_destroy(void* stream);
+int lbm_stream_sync(void* stream);
+int lbm_device_sync(void);
+int lbm_event_create(void** event);
+int lbm_event_destroy(void* event);
+int lbm_event_record(void* event, void* stream);
+int lbm_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/*
+ * Periodic ghost fill of every halo slot a pull step or a link list can read:
+ * each ghost cell receives, for every population entering the interior through
+ * all of the cell's ghost axes, the value at the fully wrapped interior
+ * position.  Equal, on those slots, to pylbm's axis-by-axis wrap of the whole
+ * halo (pylbm/storage.py:306-367; Loo.py kernels update_x/y/z 369-420).
+ */
+int lbm_ghost_fill(void* d_f, const lbm_layout* layout, const lbm_velocities* vel,
+                   void* stream);
+/* same, every population of every ghost cell (used when Neumann-type link
+ * lists read ghost slots) */
+int lbm_ghost_fill_all(void* d_f, const lbm_layout* layout, const lbm_velocities* vel,
+                       void* stream);
+
+/*
+ * One boundary link list, f[store[i]] = sign * f[load[i]] + rhs[i], where
+ * store/load are element indices into the population array (population offset
+ * included) and rhs may be NULL (all zero).  For the 16-bit deviation formats
+ * rhs must already include the rest-population shift.  Replaces the generated
+ * link loops bounce_back (pylbm/boundary.py:430-469), anti_bounce_back
+ * (646-684) and neumann / neumannx / neumanny / neumannz (787-895).
+ * two_phase != 0 gathers every load into d_work (n values of the compute type)
+ * before any store, for lists whose loads and stores overlap (the host has
+ * then already folded the sequential read-after-write chains into load/sign/rhs).
+ */
+int lbm_links_apply(void* d_f, const lbm_layout* layout, const uint32_t* d_store,
+                    const uint32_t* d_load, const void* d_rhs, const int8_t* d_sign,
+                    double sign, int64_t n, int two_phase, void* d_work, void* stream);
+
+/*
+ * FP64 compensated (Neumaier) sums of every population over the interior.
+ * d_partial needs 2*ns*(nblocks+1) doubles (nblocks from lbm_reduce_blocks),
+ * h_sum/h_comp receive ns values each; the conserved sums are
+ * sum_k M[c][k] * (h_sum[k] + h_comp[k]).  No reference counterpart: global
+ * mass / momentum monitoring of the time-stepper.
+ */
+int lbm_reduce_blocks(const lbm_layout* layout, int64_t* nblocks);
+int lbm_reduce_populations(const void* d_f, const lbm_layout* layout, const double* d_wshift,
+                           double* d_partial, double* h_sum, double* h_comp, void* stream);
+
+/* ---------------------------------------------------- generated, per scheme */
+/*
+ * Run-time parameters of the generated kernels.  omega[i] = s_i and
+ * one_minus[i] = 1 - s_i for every relaxed moment (pylbm/algorithm/base.py:404-428);
+ * wshift[k] = w_k rho0 is the rest population removed by the deviation formats.
+ */
+typedef struct lbm_params {
+  double omega[LBM_MAX_Q];
+  double one_minus[LBM_MAX_Q];
+  double wshift[LBM_MAX_Q];
+  double dt;
+  double t;
+} lbm_params;
+
+/* information on the generated module (ns, nconsm, dim) */
+int lbm_gen_info(int32_t* ns, int32_t* nconsm, int32_t* dim);
+
+/*
+ * One fused pull step on the interior: for every interior cell x,
+ * m = M f(x - v), [source dt/2], relax, [source dt/2], fnew(x) = M^-1 m.
+ * Replaces the generated one_time_step loop of
+ * pylbm/algorithm/pull.py:11-59 (registered by pylbm/algorithm/base.py:551-628).
+ * periodic_fold != 0 reads pulled values through the periodic wrap of every
+ * axis instead of the halo (valid only when no link list exists).
+ */
+int lbm_gen_step(const void* d_f, void* d_fnew, const lbm_layout* layout,
+                 const lbm_params* params, int periodic_fold, void* stream);
+
+/* f2m on the whole halo-inclusive domain into FP64 moments d_m (layout of m
+ * is `layout` with storage LBM_F64). pylbm/algorithm/base.py:298-344 */
+int lbm_gen_f2m(const void* d_f, const lbm_layout* flayout, double* d_m,
+                const lbm_layout* mlayout, const lbm_params* params, void* stream);
+/* m2f on the whole domain from FP64 moments. pylbm/algorithm/base.py:346-379 */
+int lbm_gen_m2f(const double* d_m, const lbm_layout* mlayout, void* d_f,
+                const lbm_layout* flayout, const lbm_params* params, void* stream);
+/* equilibrium in place on FP64 moments. pylbm/algorithm/base.py:381-402 */
+int lbm_gen_equilibrium(double* d_m, const lbm_layout* mlayout, void* stream);
+
+/*
+ * nsteps complete time steps (ghost fill, link lists in order, fused step,
+ * swap) without returning to the host: pylbm/simulation.py:392-420 repeated.
+ * d_f / d_fnew are swapped nsteps times; on return *parity = nsteps % 2 tells
+ * which buffer holds the newest state (0: d_f).  links: nmeth descriptors.
+ */
+typedef struct lbm_link_set {
+  const uint32_t* d_store;
+  const uint32_t* d_load;
+  const void* d_rhs;
+  const int8_t* d_sign;
+  double sign;
+  int64_t n;
+  int32_t two_phase;
+  int32_t _pad;
+  void* d_work;
+} lbm_link_set;
+
+int lbm_gen_run(void* d_f, void* d_fnew, const lbm_layout* layout, const lbm_velocities* vel,
+                lbm_params* params, const lbm_link_set* links, int nmeth, int64_t nsteps,
+                int ghost, int periodic_fold, int* parity, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYLBM_HIP_H */

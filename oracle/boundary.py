@@ -1,0 +1,123 @@
+"""Boundary link lists -- oracle restatement.
+
+Follows pylbm/boundary.py:
+  * BoundaryVelocity: x_out = x - v for fluid x whose link along -v hits
+    the label (np.where C-order over the halo-inclusive arrays)  boundary.py:23-47
+  * Boundary: labels ascending (skip -1, -2), methods in order of first
+    appearance, per method: scheme -> velocity -> C-order         boundary.py:78-160
+  * iload for BounceBack / AntiBounceBack: (ksym, x_out + v_k)     boundary.py:411-419
+  * iload for Neumann: (k, x_out + v_k); NeumannX/Y/Z shift one
+    component only                                                 boundary.py:777-890
+  * prepare_rhs: wall point x_halo[istore] + (1 - d) v_k dx, user
+    value function on moments, equilibrium, m2f                    boundary.py:238-305
+  * set_rhs: BB feq[k] - feq[ksym]; ABB feq[k] + feq[ksym]         boundary.py:421-427, 637-643
+  * loops f[istore] = +-f[iload] + rhs in link order (sequential)  boundary.py:430-464, 646-684, 787-818
+
+Pinned by: invariants checked in tests/test_oracle_boundary.py -- every
+store slot (k, x_out) has x_out + v_k fluid and flag[sym k][x_out + v_k] ==
+label, the link count equals the brute-force count of (fluid x, k) with a
+labelled neighbour, the ordering is (label, method, scheme, velocity,
+C-order) -- and bounce-back on a resting wall conserving mass per link pair.
+"""
+import numpy as np
+
+from .domain import VALIN
+from .stencil import num_to_v, v_to_num
+
+BB = "bounce_back"
+ABB = "anti_bounce_back"
+NEUMANN = "neumann"
+NEUMANN_X = "neumannx"
+NEUMANN_Y = "neumanny"
+NEUMANN_Z = "neumannz"
+
+
+class LinkMethod:
+    def __init__(self, kind, istore, ilabel, distance):
+        self.kind = kind
+        self.istore = istore          # (dim+1, n) int
+        self.ilabel = ilabel
+        self.distance = distance
+        self.iload = None
+        self.rhs = np.zeros(istore.shape[1])
+        self.feq = None
+
+
+def build_links(domain, stencil, bc_dico, method_kind):
+    """Return the list of LinkMethod (one per method class, first-appearance order).
+
+    ``method_kind(cls)`` maps a user method class to one of the kind strings.
+    """
+    dim = domain.dim
+    uv = stencil.uv
+    bv = {}
+    labels = domain.list_of_labels()
+    for label in labels:
+        if label in (-1, -2):
+            continue
+        per_u = []
+        for u in range(stencil.unvtot):
+            vs = tuple(-c for c in uv[u])
+            num = stencil.unum2index[v_to_num(vs)]
+            ind = np.array(np.where(domain.flag[num] == label))
+            if ind.size:
+                ind = ind + np.asarray(vs)[:, None]
+            per_u.append((ind, np.array(domain.distance[(num,) + tuple(np.where(domain.flag[num] == label))])))
+        bv[label] = per_u
+    order = []
+    store, lab, dist = {}, {}, {}
+    for label in labels:
+        if label in (-1, -2):
+            continue
+        methods = bc_dico[label]["method"]
+        for k, cls in methods.items():
+            kind = method_kind(cls)
+            for inumk, numk in enumerate(stencil.nums[k]):
+                ind, d = bv[label][stencil.unum2index[numk]]
+                if ind.size == 0:
+                    continue
+                n = ind.shape[1]
+                rows = np.concatenate([np.full((1, n), stencil.nv_ptr[k] + inumk), ind])
+                if kind not in store:
+                    order.append(kind)
+                    store[kind], lab[kind], dist[kind] = [], [], []
+                store[kind].append(rows)
+                lab[kind].append(np.full(n, label))
+                dist[kind].append(d)
+    out = []
+    v_all = stencil.all_velocities()
+    ksym_all = stencil.symmetric()
+    for kind in order:
+        m = LinkMethod(kind, np.concatenate(store[kind], axis=1).astype(int),
+                       np.concatenate(lab[kind]), np.concatenate(dist[kind]))
+        k = m.istore[0]
+        if kind in (BB, ABB):
+            m.iload = np.concatenate([ksym_all[k][None, :], m.istore[1:] + v_all[k].T])
+        else:
+            sh = v_all[k].T.copy()
+            if kind != NEUMANN:
+                axis = {NEUMANN_X: 0, NEUMANN_Y: 1, NEUMANN_Z: 2}[kind]
+                for d in range(dim):
+                    if d != axis:
+                        sh[d] = 0
+            m.iload = np.concatenate([k[None, :], m.istore[1:] + sh])
+        out.append(m)
+    return out
+
+
+def apply_links(f, method):
+    """f[istore] = a * f[iload] + rhs, in link order (reference loop semantics)."""
+    s = tuple(method.istore)
+    l = tuple(method.iload)
+    sign = -1.0 if method.kind == ABB else 1.0
+    add = method.rhs if method.kind in (BB, ABB) else 0.0
+    # conflict-free lists (the usual case) can be applied at once
+    st = np.ravel_multi_index(s, f.shape)
+    ld = np.ravel_multi_index(l, f.shape)
+    if np.intersect1d(st, ld).size == 0 and np.unique(st).size == st.size:
+        f[s] = sign * f[l] + add
+        return
+    flat = f.reshape(-1)
+    addv = np.broadcast_to(np.asarray(add, dtype=np.float64), st.shape)
+    for i in range(st.size):
+        flat[st[i]] = sign * flat[ld[i]] + addv[i]

@@ -1,0 +1,219 @@
+"""numpy FP64 driver for one scheme -- oracle restatement of the hot path.
+
+Follows:
+  * Simulation.initialization: init dict on the halo-inclusive meshgrid into
+    the conserved moments, then equilibrium + m2f      pylbm/simulation.py:257-320
+  * one_time_step = ghost fill, link methods in order,
+    fused pull step, swap, t += dt                     pylbm/simulation.py:373-420
+  * Array.update: periodic ghost wrap, axis by axis,
+    every population, full halo-inclusive extent       pylbm/storage.py:306-367
+  * pull step: f_k(x - v_k) -> m = M f -> [source dt/2]
+    -> relax m <- (1-s) m + s eq -> [source dt/2] ->
+    fnew = M^-1 m on the interior                      pylbm/algorithm/pull.py:11-59,
+                                                       pylbm/algorithm/base.py:298-600
+  * source half steps m_c <- m_c + dt/2 S(m)           pylbm/algorithm/ode.py:11-16,
+                                                       pylbm/algorithm/base.py:439-495
+  * read-back m = M f on the full domain, interior
+    view                                               pylbm/simulation.py:215-224
+
+Pinned by the known answers in tests/test_oracle_lbm.py: the rest state is a
+fixed point; with every non-conserved rate 0 a step is the exact permutation
+f_k(x, t+1) = f_k(x - v_k, t) on a periodic box; with every rate 1 the
+post-collision state is M^-1 EQ(m_c); mass and momentum are conserved to
+round-off on periodic boxes (math.fsum); bounce-back on resting walls
+conserves mass.  The C1-C5 trajectories themselves have no reference
+fixture, so this driver's end-to-end output is "parity unpinned" beyond
+those checks.
+"""
+import math
+
+import numpy as np
+
+from .boundary import ABB, BB, NEUMANN, NEUMANN_X, NEUMANN_Y, NEUMANN_Z, apply_links, build_links
+from .domain import Domain
+from .scheme import Scheme
+
+_KINDS = {"BounceBack": BB, "AntiBounceBack": ABB, "Neumann": NEUMANN,
+          "NeumannX": NEUMANN_X, "NeumannY": NEUMANN_Y, "NeumannZ": NEUMANN_Z}
+
+
+def method_kind(cls):
+    name = cls if isinstance(cls, str) else cls.__name__
+    return _KINDS[name]
+
+
+class _Moments:
+    """Minimal moment container for user value/init functions (symbol keys)."""
+
+    def __init__(self, arr, consm):
+        self.arr = arr
+        self.consm = consm
+
+    def _key(self, key):
+        return self.consm[key] if not isinstance(key, (int, np.integer)) else key
+
+    def __getitem__(self, key):
+        return self.arr[self._key(key)]
+
+    def __setitem__(self, key, value):
+        self.arr[self._key(key)] = value
+
+
+class Simulation:
+    def __init__(self, dico):
+        self.dico = dico
+        self.domain = Domain(dico)
+        self.scheme = Scheme(dico)
+        st = self.scheme.stencil
+        self.dim = self.domain.dim
+        self.ns = self.scheme.ns
+        self.v = st.all_velocities()
+        self.h = np.asarray(st.vmax)
+        self.dx = self.domain.dx
+        self.dt = self.dx / self.scheme.la
+        self.t = 0.0
+        self.nt = 0
+        shape = [self.ns] + list(self.domain.shape_halo)
+        self.F = np.zeros(shape)
+        self.Fnew = np.zeros(shape)
+        self.methods = build_links(self.domain, st, dico.get("boundary_conditions", {}), method_kind)
+        self._initialize()
+        for m in self.methods:
+            self._prepare_rhs(m)
+
+    # ------------------------------------------------------------ helpers
+    def _eq_full(self, m):
+        nc = self.scheme.nconsm
+        return np.stack(self.scheme.equilibrium([m[i] for i in range(nc)]))
+
+    def _m2f(self, m):
+        return np.tensordot(self.scheme.invMnum, m, axes=(1, 0))
+
+    def _f2m(self, f):
+        return np.tensordot(self.scheme.Mnum, f, axes=(1, 0))
+
+    def _initialize(self):
+        coords = np.meshgrid(*self.domain.coords_halo, sparse=True, indexing="ij")
+        m = np.zeros_like(self.F)
+        mv = _Moments(m, self.scheme.consm)
+        init = self.dico.get("init", {})
+        for k, v in init.items():
+            if isinstance(v, tuple):
+                mv[k] = v[0](*(tuple(coords) + (tuple(v[1]) if len(v) == 2 else ())))
+            elif callable(v):
+                mv[k] = v(*coords)
+            else:
+                mv[k] = v
+        if self.dico.get("inittype", "moments") == "moments":
+            m = self._eq_full(m)
+            self.F[...] = self._m2f(m)
+        else:
+            raise NotImplementedError("oracle: distributions init")
+        self.Fnew[...] = self.F
+
+    def _prepare_rhs(self, meth):
+        bcd = self.dico["boundary_conditions"]
+        meth.feq = np.zeros((self.ns, meth.istore.shape[1]))
+        for key in np.unique(meth.ilabel):
+            value = bcd[key].get("value")
+            if value is None:
+                continue
+            if bcd[key].get("time_bc", False):
+                raise NotImplementedError("oracle: time-dependent boundary values")
+            idx = np.where(meth.ilabel == key)
+            n = idx[0].size
+            k = meth.istore[0, idx]
+            s = 1 - meth.distance[idx]
+            coords = ()
+            for i in range(self.dim):
+                x = self.domain.coords_halo[i][meth.istore[i + 1, idx]]
+                x += s * self.v[k, i] * self.dx
+                x = x.ravel()
+                for _ in range(1, self.dim):
+                    x = x[:, np.newaxis]
+                coords += (x,)
+            shp = [self.ns, n] + [1] * (self.dim - 1)
+            m = np.zeros(shp)
+            f = np.zeros(shp)
+            func, args = (value[0], tuple(value[1])) if isinstance(value, tuple) else (value, ())
+            func(f, _Moments(m, self.scheme.consm), *(coords + args))
+            m = self._eq_full(m)
+            f = self._m2f(m)
+            meth.feq[:, idx[0]] = f.reshape(self.ns, n)
+        k = meth.istore[0]
+        if meth.kind in (BB, ABB):
+            ksym = self.scheme.stencil.symmetric()[k]
+            cols = np.arange(k.size)
+            sign = -1.0 if meth.kind == BB else 1.0
+            meth.rhs = meth.feq[k, cols] + sign * meth.feq[ksym, cols]
+
+    # ------------------------------------------------------------ one step
+    def ghost_fill(self, f):
+        for d in range(self.dim):
+            h = int(self.h[d])
+            if h == 0:
+                continue
+            n = f.shape[d + 1]
+            lo = [slice(None)] * f.ndim
+            src = [slice(None)] * f.ndim
+            lo[d + 1] = slice(0, h)
+            src[d + 1] = slice(n - 2 * h, n - h)
+            f[tuple(lo)] = f[tuple(src)]
+            lo[d + 1] = slice(n - h, n)
+            src[d + 1] = slice(h, 2 * h)
+            f[tuple(lo)] = f[tuple(src)]
+
+    def _interior(self, shift=None):
+        sl = [slice(None)]
+        for d in range(self.dim):
+            h = int(self.h[d])
+            n = self.domain.shape_halo[d]
+            s = 0 if shift is None else int(shift[d])
+            sl.append(slice(h - s, n - h - s))
+        return tuple(sl)
+
+    def pull_step(self, f, fnew):
+        ns = self.ns
+        shp = [ns] + list(self.domain.shape_in)
+        fp = np.empty(shp)
+        for k in range(ns):
+            fp[k] = f[(k,) + self._interior(self.v[k])[1:]]
+        m = self._f2m(fp)
+        self._sources(m)
+        eq = self._eq_full(m)
+        for i in range(ns):
+            s = self.scheme.s[i]
+            if s != 0:
+                m[i] = (1 - s) * m[i] + s * eq[i]
+        self._sources(m)
+        fnew[self._interior()] = self._m2f(m)
+
+    def _sources(self, m):
+        if not self.scheme.sources:
+            return
+        coords = np.meshgrid(*self.domain.coords, sparse=True, indexing="ij")
+        for idx, fn in self.scheme.sources:
+            m[idx] = m[idx] + 0.5 * self.dt * fn(list(m), self.t, *coords)
+
+    def one_time_step(self):
+        self.ghost_fill(self.F)
+        for meth in self.methods:
+            apply_links(self.F, meth)
+        self.pull_step(self.F, self.Fnew)
+        self.F, self.Fnew = self.Fnew, self.F
+        self.t += self.dt
+        self.nt += 1
+
+    # ------------------------------------------------------------ read-back
+    def moments(self):
+        """All moments on the interior (ns, *shape_in)."""
+        return self._f2m(self.F[self._interior()])
+
+    def m(self, key):
+        i = self.scheme.consm[key] if not isinstance(key, (int, np.integer)) else key
+        return self.moments()[i]
+
+    def conserved_sums(self):
+        """math.fsum of every conserved moment over the interior."""
+        mm = self.moments()
+        return [math.fsum(mm[i].ravel()) for i in range(self.scheme.nconsm)]

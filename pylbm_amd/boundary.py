@@ -1,0 +1,247 @@
+"""Boundary link lists and their device form.
+
+Host side mirrors pylbm/boundary.py:
+  * ``Boundary``: labels ascending (periodic -1 / interface -2 skipped), one
+    method instance per method class in order of first appearance, links of
+    each method concatenated label -> scheme -> velocity -> C-order of the
+    fluid cells, store slot (k, x_out = x - v_k)            boundary.py:23-160
+  * ``set_iload`` per method (bounce-back family loads (ksym, x_out + v_k),
+    Neumann loads (k, x_out + v_k) or one component of it) boundary.py:411-419, 777-890
+  * ``prepare_rhs``: wall points x_out + (1 - d) v_k dx, user value function
+    on moments, equilibrium, m2f (here the generated GPU kernels)
+                                                            boundary.py:238-305
+  * ``set_rhs``: BB feq_k - feq_ksym, ABB feq_k + feq_ksym   boundary.py:421-427, 637-643
+  * ``fix_iload``: int32 (n, dim + 1) rows                   boundary.py:226-235
+
+Device side: each method becomes flat uint32 element indices (store, load)
+into the population array plus a compute-type rhs (omitted when zero), run
+by ``lbm_links_apply``.  The reference loop is sequential; when a method's
+loads hit its own stores (possible with Neumann on adjacent faces) the
+read-after-write chains are folded on the host and the list runs in its
+two-phase (gather all, then scatter) form, which reproduces the sequential
+result exactly.
+"""
+import collections
+import ctypes
+
+import numpy as np
+
+from . import bc as bcmod
+from .runtime import LBM_F16D, LBM_BF16D, LBM_F64, LbmLinkSet
+
+__all__ = ["Boundary", "BoundaryMethod"]
+
+
+class BoundaryMethod:
+    """Link list of one method class (pylbm/boundary.py:164-397)."""
+
+    def __init__(self, kind, istore, ilabel, distance, stencil, value_bc, time_bc):
+        self.kind = kind
+        self.name = kind.name
+        self.istore = istore                 # (dim + 1, n) before fix_iload
+        self.ilabel = ilabel
+        self.distance = distance
+        self.stencil = stencil
+        self.feq = np.zeros((int(stencil.nv_ptr[-1]), istore.shape[1]))
+        self.rhs = np.zeros(istore.shape[1])
+        self.value_bc = {k: value_bc[k] for k in np.unique(ilabel)}
+        self.time_bc = {k: time_bc[k] for k in np.unique(ilabel)}
+        self.iload = []
+        self._fixed = False
+        self.device = None
+        self._tbc = []
+
+    @property
+    def ncond(self):
+        return self.istore.shape[1] if not self._fixed else self.istore.shape[0]
+
+    def set_iload(self):
+        k = self.istore[0]
+        v = self.stencil.get_all_velocities()
+        if self.kind.symmetric_load:
+            ksym = self.stencil.get_symmetric()[k][np.newaxis, :]
+            self.iload.append(np.concatenate([ksym, self.istore[1:] + v[k].T]))
+        else:
+            shift = v[k].T.copy()
+            if self.kind.shift_axis is not None:
+                keep = shift[self.kind.shift_axis].copy()
+                shift[:] = 0
+                shift[self.kind.shift_axis] = keep
+            self.iload.append(np.concatenate([k[np.newaxis, :], self.istore[1:] + shift]))
+
+    def fix_iload(self):
+        self.iload = [np.ascontiguousarray(a.T, dtype=np.int32) for a in self.iload]
+        self.istore = np.ascontiguousarray(self.istore.T, dtype=np.int32)
+        self._fixed = True
+
+    def _store_rows(self):
+        return self.istore.T if self._fixed else self.istore
+
+    def set_rhs(self):
+        if self.kind.rhs_sign is None:
+            return
+        k = self._store_rows()[0]
+        ksym = self.stencil.get_symmetric()[k]
+        cols = np.arange(k.size)
+        self.rhs[:] = self.feq[k, cols] + self.kind.rhs_sign * self.feq[ksym, cols]
+
+
+class Boundary:
+    """All link-list methods of a simulation (pylbm/boundary.py:50-160)."""
+
+    def __init__(self, domain, dico):
+        self.domain = domain
+        stencil = domain.stencil
+        dico_bound = dico.get("boundary_conditions", {}) or {}
+        links = domain.links()
+        store = collections.OrderedDict()
+        labs, dists = {}, {}
+        value_bc, time_bc = {}, {}
+        shape = domain.shape_halo
+        for label in domain.list_of_labels():
+            if label in (-1, -2):
+                continue
+            if label not in dico_bound:
+                raise KeyError("no boundary condition for label %s" % label)
+            value_bc[label] = dico_bound[label].get("value", None)
+            time_bc[label] = dico_bound[label].get("time_bc", False)
+            for k, cls in dico_bound[label]["method"].items():
+                if getattr(cls, "unsupported", False) or not issubclass(cls, bcmod.BoundaryMethodKind):
+                    raise NotImplementedError("boundary method %s is not provided by the hip generator"
+                                              % getattr(cls, "__name__", cls))
+                for inumk, numk in enumerate(stencil.num(k)):
+                    u = int(stencil.unum2index[numk])
+                    key = (label, domain.sym_index(u))
+                    if key not in links:
+                        continue
+                    pos, dist = links[key]
+                    x = np.array(np.unravel_index(pos, shape))
+                    x_out = x - np.asarray(stencil.uv[u])[:, None]
+                    rows = np.concatenate([np.full((1, pos.size), stencil.nv_ptr[k] + inumk), x_out])
+                    store.setdefault(cls, []).append(rows)
+                    labs.setdefault(cls, []).append(np.full(pos.size, label))
+                    dists.setdefault(cls, []).append(dist)
+        self.methods = []
+        for cls in store:
+            m = BoundaryMethod(cls, np.concatenate(store[cls], axis=1).astype(np.int64),
+                               np.concatenate(labs[cls]), np.concatenate(dists[cls]),
+                               stencil, value_bc, time_bc)
+            self.methods.append(m)
+
+
+# --------------------------------------------------------------- device form
+def _flat(rows, layout):
+    """Element indices of (k, x...) rows (halo-inclusive reference axes) in the device layout."""
+    dim = rows.shape[0] - 1
+    c = [np.zeros(rows.shape[1], dtype=np.int64)] * (3 - dim) + [rows[1 + d].astype(np.int64) for d in range(dim)]
+    idx = (rows[0].astype(np.int64) * layout.pop_stride
+           + (c[0] * layout.n[1] + c[1]) * layout.pitch + layout.lead + c[2])
+    if idx.size and (idx.min() < 0 or idx.max() >= (1 << 32)):
+        raise ValueError("link index out of the 32-bit range")
+    return idx
+
+
+def _resolve_sequential(store, load, sign, rhs):
+    """Fold read-after-write chains of a sequential loop f[s_i] = a_i f[l_i] + r_i.
+
+    Returns (store, load, sign, rhs) such that gathering every load before any
+    store gives the sequential result; earlier duplicate stores are dropped."""
+    written = {}
+    out_l = np.empty_like(load)
+    out_a = np.empty(load.size, dtype=np.float64)
+    out_r = np.empty(load.size, dtype=np.float64)
+    for i in range(store.size):
+        l = int(load[i])
+        a, r = float(sign[i]), float(rhs[i])
+        if l in written:
+            aw, lw, rw = written[l]
+            a, l, r = a * aw, lw, a * rw + r
+        written[int(store[i])] = (a, l, r)
+        out_l[i], out_a[i], out_r[i] = l, a, r
+    last = {}
+    for i in range(store.size):
+        last[int(store[i])] = i
+    keep = np.array(sorted(last.values()), dtype=np.int64)
+    return store[keep], out_l[keep], out_a[keep], out_r[keep]
+
+
+class DeviceLinks:
+    """Device arrays of one method for a given population layout."""
+
+    def __init__(self, rt, method, layout, wshift):
+        self.rt = rt
+        self.method = method
+        self.layout = layout
+        rows_s = method._store_rows()
+        rows_l = method.iload[0].T if method._fixed else method.iload[0]
+        st = _flat(rows_s, layout)
+        ld = _flat(rows_l, layout)
+        self.n = st.size
+        sign = np.full(self.n, method.kind.sign)
+        rhs = method.rhs.copy() if method.kind.rhs_sign is not None else np.zeros(self.n)
+        self.conflict = (np.intersect1d(st, ld).size > 0) or (np.unique(st).size != st.size)
+        if self.conflict:
+            st, ld, sign, rhs = _resolve_sequential(st, ld, sign, rhs)
+            self.n = st.size
+        kstore = (st // layout.pop_stride).astype(np.int64)
+        kload = (ld // layout.pop_stride).astype(np.int64)
+        if layout.storage in (LBM_F16D, LBM_BF16D):
+            # deviations: d_s = a d_l + (r + a w_l - w_s)
+            rhs = rhs + sign * wshift[kload] - wshift[kstore]
+        self.rhs_host = rhs
+        ctype = np.float64 if layout.storage == LBM_F64 else np.float32
+        self._ptrs = []
+        self.d_store = self._up(st.astype(np.uint32))
+        self.d_load = self._up(ld.astype(np.uint32))
+        self.d_rhs = self._up(rhs.astype(ctype)) if np.any(rhs != 0) else None
+        uniform = np.all(sign == sign[0]) if self.n else True
+        self.sign = float(sign[0]) if self.n else 1.0
+        self.d_sign = None if uniform else self._up(sign.astype(np.int8))
+        self.d_work = rt.malloc(max(1, self.n) * np.dtype(ctype).itemsize) if self.conflict else None
+        if self.d_work:
+            self._ptrs.append(self.d_work)
+
+    def _up(self, a):
+        p = self.rt.upload(a)
+        self._ptrs.append(p)
+        return p
+
+    def update_rhs(self, rhs, wshift):
+        """Re-upload rhs (time-dependent boundary values)."""
+        if self.conflict:
+            raise NotImplementedError("time-dependent values on a self-overlapping link list")
+        if self.layout.storage in (LBM_F16D, LBM_BF16D):
+            st = _flat(self.method._store_rows(), self.layout)
+            ld = _flat(self.method.iload[0].T if self.method._fixed else self.method.iload[0], self.layout)
+            rhs = rhs + self.method.kind.sign * wshift[ld // self.layout.pop_stride] - wshift[st // self.layout.pop_stride]
+        ctype = np.float64 if self.layout.storage == LBM_F64 else np.float32
+        if self.d_rhs is None:
+            self.d_rhs = self._up(rhs.astype(ctype))
+        else:
+            self.rt.h2d(self.d_rhs, rhs.astype(ctype))
+
+    def link_set(self):
+        s = LbmLinkSet()
+        s.d_store, s.d_load = self.d_store, self.d_load
+        s.d_rhs = self.d_rhs
+        s.d_sign = self.d_sign
+        s.sign = self.sign
+        s.n = self.n
+        s.two_phase = 1 if self.conflict else 0
+        s.d_work = self.d_work
+        return s
+
+    def apply(self, d_f):
+        rt = self.rt
+        rt.check(rt.lib.lbm_links_apply(ctypes.c_void_p(d_f), ctypes.byref(self.layout),
+                                        ctypes.c_void_p(self.d_store), ctypes.c_void_p(self.d_load),
+                                        ctypes.c_void_p(self.d_rhs), ctypes.c_void_p(self.d_sign),
+                                        self.sign, self.n, 1 if self.conflict else 0,
+                                        ctypes.c_void_p(self.d_work), rt.stream), self.method.name)
+
+    def __del__(self):
+        try:
+            for p in self._ptrs:
+                self.rt.free(p)
+        except Exception:
+            pass

@@ -1,0 +1,146 @@
+// Device-side helpers shared by the core runtime and the generated scheme
+// kernels: population storage codecs, layout arithmetic, error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "pylbm_hip.h"
+
+namespace lbm {
+
+// ------------------------------------------------------------------ errors
+// One message per host thread; set by every failing entry point.
+void set_error(const char* fmt, ...);
+
+#define LBM_CHECK_HIP(expr)                                                  \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ::lbm::set_error("%s failed: %s", #expr, hipGetErrorString(e_));       \
+      return (int)e_;                                                        \
+    }                                                                        \
+  } while (0)
+
+#define LBM_REQUIRE(cond, ...)                                               \
+  do {                                                                       \
+    if (!(cond)) {                                                           \
+      ::lbm::set_error(__VA_ARGS__);                                         \
+      return -1;                                                             \
+    }                                                                        \
+  } while (0)
+
+// ------------------------------------------------------------ bfloat16 bits
+struct bf16 {
+  uint16_t bits;
+};
+
+__device__ __forceinline__ float bf16_to_float(bf16 h) {
+  return __uint_as_float(((uint32_t)h.bits) << 16);
+}
+
+// round to nearest even (no NaN payload handling: populations are finite)
+__device__ __forceinline__ bf16 float_to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  bf16 r;
+  r.bits = (uint16_t)(u >> 16);
+  return r;
+}
+
+// ------------------------------------------------------------ storage codec
+// Codec<S>::C is the compute type, load/store convert a stored population
+// to/from it; w is the rest population w_k rho0 removed by the deviation
+// formats (ignored by the full-value formats).
+template <typename S>
+struct Codec;
+
+template <>
+struct Codec<double> {
+  using C = double;
+  static constexpr int kind = LBM_F64;
+  __device__ __forceinline__ static C load(const double* p, int64_t i, C) { return p[i]; }
+  __device__ __forceinline__ static void store(double* p, int64_t i, C v, C) { p[i] = v; }
+  __device__ __forceinline__ static double widen(double s, double) { return s; }
+};
+
+template <>
+struct Codec<float> {
+  using C = float;
+  static constexpr int kind = LBM_F32;
+  __device__ __forceinline__ static C load(const float* p, int64_t i, C) { return p[i]; }
+  __device__ __forceinline__ static void store(float* p, int64_t i, C v, C) { p[i] = v; }
+  __device__ __forceinline__ static double widen(float s, double) { return (double)s; }
+};
+
+template <>
+struct Codec<_Float16> {
+  using C = float;
+  static constexpr int kind = LBM_F16D;
+  __device__ __forceinline__ static C load(const _Float16* p, int64_t i, C w) {
+    return w + (float)p[i];
+  }
+  __device__ __forceinline__ static void store(_Float16* p, int64_t i, C v, C w) {
+    p[i] = (_Float16)(v - w);
+  }
+  __device__ __forceinline__ static double widen(_Float16 s, double w) { return w + (double)(float)s; }
+};
+
+template <>
+struct Codec<bf16> {
+  using C = float;
+  static constexpr int kind = LBM_BF16D;
+  __device__ __forceinline__ static C load(const bf16* p, int64_t i, C w) {
+    return w + bf16_to_float(p[i]);
+  }
+  __device__ __forceinline__ static void store(bf16* p, int64_t i, C v, C w) {
+    p[i] = float_to_bf16(v - w);
+  }
+  __device__ __forceinline__ static double widen(bf16 s, double w) {
+    return w + (double)bf16_to_float(s);
+  }
+};
+
+inline int storage_elem_size(int storage) {
+  switch (storage) {
+    case LBM_F64: return 8;
+    case LBM_F32: return 4;
+    case LBM_F16D: return 2;
+    case LBM_BF16D: return 2;
+    default: return 0;
+  }
+}
+
+// interior extents of the canonical axes
+__host__ __device__ __forceinline__ int64_t interior(const lbm_layout& L, int a) {
+  return L.n[a] - 2 * L.halo[a];
+}
+
+// element offset of (o, m, i) inside one population (halo-inclusive indices)
+__host__ __device__ __forceinline__ int64_t cell_offset(const lbm_layout& L, int64_t o, int64_t m,
+                                                        int64_t i) {
+  return (o * L.n[1] + m) * L.pitch + L.lead + i;
+}
+
+inline int check_layout(const lbm_layout* L) {
+  LBM_REQUIRE(L != nullptr, "layout is NULL");
+  LBM_REQUIRE(L->ns > 0 && L->ns <= LBM_MAX_Q, "ns=%d out of range [1, %d]", L->ns, LBM_MAX_Q);
+  LBM_REQUIRE(L->dim >= 1 && L->dim <= 3, "dim=%d out of range", L->dim);
+  LBM_REQUIRE(storage_elem_size(L->storage) == L->elem_size, "storage %d / elem_size %d mismatch",
+              L->storage, L->elem_size);
+  for (int a = 0; a < 3; ++a) {
+    LBM_REQUIRE(L->n[a] >= 1 && L->halo[a] >= 0 && L->n[a] > 2 * L->halo[a],
+                "axis %d: extent %lld halo %d invalid", a, (long long)L->n[a], L->halo[a]);
+  }
+  LBM_REQUIRE(L->lead >= 0 && L->lead + L->n[2] <= L->pitch, "row lead/pitch too small");
+  LBM_REQUIRE(L->pop_stride >= L->n[0] * L->n[1] * L->pitch, "pop_stride too small");
+  // every element index must fit the 32-bit offsets used by the kernels
+  LBM_REQUIRE(L->pop_stride * (int64_t)L->ns < ((int64_t)1 << 31),
+              "array of %lld elements exceeds the 2^31 element index range",
+              (long long)(L->pop_stride * (int64_t)L->ns));
+  return 0;
+}
+
+}  // namespace lbm

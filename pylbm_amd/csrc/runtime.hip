@@ -1,0 +1,449 @@
+// Core runtime of the MI355X lattice Boltzmann time-stepper (libpylbm_hip.so):
+// device memory, copies, streams, events, and the scheme-independent kernels
+// (periodic ghost fill, boundary link lists, FP64 compensated population sums).
+#include <stdarg.h>
+
+#include "lbm_device.h"
+
+namespace lbm {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// ------------------------------------------------------------- ghost fill
+// One thread per ghost cell of the halo shell, partitioned into
+//   A: outer-axis ghost slabs (full mid/inner extent)
+//   B: mid-axis ghost slabs (interior outer, full inner)
+//   C: inner-axis ghost slabs (interior outer and mid)
+// For every population that enters the interior through all the ghost axes of
+// the cell (component >= depth on the low side, <= -depth on the high side) the
+// value at the fully wrapped interior position is copied.  all_pops copies every
+// population instead (the full axis-by-axis wrap).
+struct ShellDesc {
+  int64_t nA, nB, nC;
+};
+
+__device__ __forceinline__ bool ghost_axis(int64_t c, int64_t n, int h, int64_t* wrapped,
+                                           int* depth) {
+  const int64_t nin = n - 2 * h;
+  if (c < h) {
+    *wrapped = c + nin;
+    *depth = (int)(h - c);  // > 0: low side
+    return true;
+  }
+  if (c >= n - h) {
+    *wrapped = c - nin;
+    *depth = -(int)(c - (n - h) + 1);  // < 0: high side
+    return true;
+  }
+  *wrapped = c;
+  *depth = 0;
+  return false;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) ghost_fill_kernel(T* __restrict__ f, lbm_layout L,
+                                                         lbm_velocities V, ShellDesc S,
+                                                         int all_pops) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = S.nA + S.nB + S.nC;
+  if (t >= total) return;
+  const int64_t n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+  const int h0 = L.halo[0], h1 = L.halo[1], h2 = L.halo[2];
+  int64_t o, m, i;
+  if (t < S.nA) {  // outer ghost slabs: 2*h0 x n1 x n2
+    int64_t r = t;
+    i = r % n2;
+    r /= n2;
+    m = r % n1;
+    r /= n1;
+    o = (r < h0) ? r : (n0 - 2 * h0 + r);
+  } else if (t < S.nA + S.nB) {  // mid ghost slabs: (n0-2h0) x 2*h1 x n2
+    int64_t r = t - S.nA;
+    i = r % n2;
+    r /= n2;
+    const int64_t g = r % (2 * h1);
+    r /= (2 * h1);
+    o = h0 + r;
+    m = (g < h1) ? g : (n1 - 2 * h1 + g);
+  } else {  // inner ghost slabs: (n0-2h0) x (n1-2h1) x 2*h2
+    int64_t r = t - S.nA - S.nB;
+    const int64_t g = r % (2 * h2);
+    r /= (2 * h2);
+    m = h1 + r % (n1 - 2 * h1);
+    o = h0 + r / (n1 - 2 * h1);
+    i = (g < h2) ? g : (n2 - 2 * h2 + g);
+  }
+  int64_t wo, wm, wi;
+  int d0, d1, d2;
+  ghost_axis(o, n0, h0, &wo, &d0);
+  ghost_axis(m, n1, h1, &wm, &d1);
+  ghost_axis(i, n2, h2, &wi, &d2);
+  const int64_t dst = cell_offset(L, o, m, i);
+  const int64_t src = cell_offset(L, wo, wm, wi);
+  for (int k = 0; k < L.ns; ++k) {
+    bool take = true;
+    if (!all_pops) {
+      const int c0 = V.v[k][0], c1 = V.v[k][1], c2 = V.v[k][2];
+      take = (d0 == 0 || (d0 > 0 ? c0 >= d0 : c0 <= d0)) &&
+             (d1 == 0 || (d1 > 0 ? c1 >= d1 : c1 <= d1)) &&
+             (d2 == 0 || (d2 > 0 ? c2 >= d2 : c2 <= d2));
+    }
+    if (take) f[k * L.pop_stride + dst] = f[k * L.pop_stride + src];
+  }
+}
+
+// ------------------------------------------------------------ link lists
+template <typename S>
+__global__ void __launch_bounds__(256) links_kernel(S* __restrict__ f,
+                                                    const uint32_t* __restrict__ store,
+                                                    const uint32_t* __restrict__ load,
+                                                    const typename Codec<S>::C* __restrict__ rhs,
+                                                    const int8_t* __restrict__ sgn, float sign,
+                                                    int64_t n) {
+  using C = typename Codec<S>::C;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  // deviation formats: rhs already carries the rest-population shift, so the
+  // stored deviations are combined directly (w = 0)
+  C v = Codec<S>::load(f, load[t], (C)0);
+  const C a = sgn ? (C)sgn[t] : (C)sign;
+  C r = a * v;
+  if (rhs) r += rhs[t];
+  Codec<S>::store(f, store[t], r, (C)0);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) links_gather_kernel(const S* __restrict__ f,
+                                                           const uint32_t* __restrict__ load,
+                                                           const typename Codec<S>::C* __restrict__ rhs,
+                                                           const int8_t* __restrict__ sgn, float sign,
+                                                           int64_t n,
+                                                           typename Codec<S>::C* __restrict__ work) {
+  using C = typename Codec<S>::C;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const C a = sgn ? (C)sgn[t] : (C)sign;
+  C r = a * Codec<S>::load(f, load[t], (C)0);
+  if (rhs) r += rhs[t];
+  work[t] = r;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) links_scatter_kernel(S* __restrict__ f,
+                                                            const uint32_t* __restrict__ store,
+                                                            int64_t n,
+                                                            const typename Codec<S>::C* __restrict__ work) {
+  using C = typename Codec<S>::C;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  Codec<S>::store(f, store[t], work[t], (C)0);
+}
+
+template <typename S>
+static int links_launch(void* d_f, const uint32_t* st, const uint32_t* ld, const void* rhs,
+                        const int8_t* sgn, double sign, int64_t n, int two_phase, void* work,
+                        hipStream_t s) {
+  using C = typename Codec<S>::C;
+  const int64_t blocks = (n + 255) / 256;
+  if (!two_phase) {
+    links_kernel<S><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+        (S*)d_f, st, ld, (const C*)rhs, sgn, (float)sign, n);
+  } else {
+    LBM_REQUIRE(work != nullptr, "two-phase link list needs a workspace");
+    links_gather_kernel<S><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+        (const S*)d_f, ld, (const C*)rhs, sgn, (float)sign, n, (C*)work);
+    links_scatter_kernel<S><<<dim3((unsigned)blocks), dim3(256), 0, s>>>((S*)d_f, st, n,
+                                                                        (const C*)work);
+  }
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------- compensated sums
+// Neumaier accumulation per thread, TwoSum-based pairwise combination of
+// (sum, compensation) pairs across the block and then across blocks.
+struct Acc {
+  double s, c;
+};
+
+__device__ __forceinline__ void neumaier_add(Acc& a, double x) {
+  const double t = a.s + x;
+  if (fabs(a.s) >= fabs(x))
+    a.c += (a.s - t) + x;
+  else
+    a.c += (x - t) + a.s;
+  a.s = t;
+}
+
+__device__ __forceinline__ Acc acc_merge(Acc a, Acc b) {
+  Acc r = a;
+  neumaier_add(r, b.s);
+  r.c += b.c;
+  return r;
+}
+
+constexpr int kRedThreads = 256;
+
+template <typename S>
+__global__ void __launch_bounds__(kRedThreads) reduce_pop_kernel(const S* __restrict__ f,
+                                                                 lbm_layout L,
+                                                                 const double* __restrict__ wshift,
+                                                                 double* __restrict__ partial) {
+  // grid: (nblocks, ns); rows (o, m) of the interior are distributed over blocks
+  const int k = blockIdx.y;
+  const int64_t nrows = interior(L, 0) * interior(L, 1);
+  const int64_t nin = interior(L, 2);
+  const double w = wshift ? wshift[k] : 0.0;
+  Acc a{0.0, 0.0};
+  const S* fk = f + k * L.pop_stride;
+  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const int64_t o = L.halo[0] + r / interior(L, 1);
+    const int64_t m = L.halo[1] + r % interior(L, 1);
+    const int64_t base = cell_offset(L, o, m, L.halo[2]);
+    for (int64_t i = threadIdx.x; i < nin; i += blockDim.x)
+      neumaier_add(a, Codec<S>::widen(fk[base + i], w));
+  }
+  __shared__ Acc sh[kRedThreads];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] = acc_merge(sh[threadIdx.x], sh[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * ((int64_t)k * gridDim.x + blockIdx.x)] = sh[0].s;
+    partial[2 * ((int64_t)k * gridDim.x + blockIdx.x) + 1] = sh[0].c;
+  }
+}
+
+__global__ void __launch_bounds__(kRedThreads) reduce_final_kernel(double* __restrict__ partial,
+                                                                   int64_t nblocks,
+                                                                   double* __restrict__ out) {
+  const int k = blockIdx.x;
+  Acc a{0.0, 0.0};
+  for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    Acc p{partial[2 * (k * nblocks + b)], partial[2 * (k * nblocks + b) + 1]};
+    a = acc_merge(a, p);
+  }
+  __shared__ Acc sh[kRedThreads];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] = acc_merge(sh[threadIdx.x], sh[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * k] = sh[0].s;
+    out[2 * k + 1] = sh[0].c;
+  }
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+// ======================================================================= C ABI
+extern "C" {
+
+const char* lbm_last_error(void) { return g_err; }
+
+int lbm_device_count(int* count) {
+  LBM_REQUIRE(count, "count is NULL");
+  LBM_CHECK_HIP(hipGetDeviceCount(count));
+  return 0;
+}
+
+int lbm_set_device(int device) {
+  LBM_CHECK_HIP(hipSetDevice(device));
+  return 0;
+}
+
+int lbm_device_name(int device, char* buf, int len) {
+  hipDeviceProp_t p;
+  LBM_CHECK_HIP(hipGetDeviceProperties(&p, device));
+  snprintf(buf, (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+  return 0;
+}
+
+int lbm_malloc(void** d_ptr, uint64_t bytes) {
+  LBM_REQUIRE(d_ptr, "d_ptr is NULL");
+  *d_ptr = nullptr;
+  LBM_CHECK_HIP(hipMalloc(d_ptr, bytes ? bytes : 256));
+  return 0;
+}
+
+int lbm_free(void* d_ptr) {
+  if (d_ptr) LBM_CHECK_HIP(hipFree(d_ptr));
+  return 0;
+}
+
+int lbm_memcpy_h2d(void* d_dst, const void* h_src, uint64_t bytes, void* stream) {
+  if (!bytes) return 0;
+  LBM_CHECK_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  LBM_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+int lbm_memcpy_d2h(void* h_dst, const void* d_src, uint64_t bytes, void* stream) {
+  if (!bytes) return 0;
+  LBM_CHECK_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  LBM_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+int lbm_memcpy_d2d(void* d_dst, const void* d_src, uint64_t bytes, void* stream) {
+  if (!bytes) return 0;
+  LBM_CHECK_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return 0;
+}
+
+int lbm_memset(void* d_ptr, int value, uint64_t bytes, void* stream) {
+  if (!bytes) return 0;
+  LBM_CHECK_HIP(hipMemsetAsync(d_ptr, value, bytes, (hipStream_t)stream));
+  return 0;
+}
+
+int lbm_stream_create(void** stream) {
+  LBM_REQUIRE(stream, "stream is NULL");
+  hipStream_t s;
+  LBM_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void*)s;
+  return 0;
+}
+
+int lbm_stream_destroy(void* stream) {
+  if (stream) LBM_CHECK_HIP(hipStreamDestroy((hipStream_t)stream));
+  return 0;
+}
+
+int lbm_stream_sync(void* stream) {
+  LBM_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+int lbm_device_sync(void) {
+  LBM_CHECK_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+int lbm_event_create(void** event) {
+  LBM_REQUIRE(event, "event is NULL");
+  hipEvent_t e;
+  LBM_CHECK_HIP(hipEventCreate(&e));
+  *event = (void*)e;
+  return 0;
+}
+
+int lbm_event_destroy(void* event) {
+  if (event) LBM_CHECK_HIP(hipEventDestroy((hipEvent_t)event));
+  return 0;
+}
+
+int lbm_event_record(void* event, void* stream) {
+  LBM_CHECK_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return 0;
+}
+
+int lbm_event_elapsed_ms(void* start, void* stop, float* ms) {
+  LBM_CHECK_HIP(hipEventSynchronize((hipEvent_t)stop));
+  LBM_CHECK_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return 0;
+}
+
+static int ghost_fill_impl(void* d_f, const lbm_layout* L, const lbm_velocities* V, int all_pops,
+                           void* stream) {
+  int rc = check_layout(L);
+  if (rc) return rc;
+  LBM_REQUIRE(V && V->q == L->ns, "velocity table does not match the layout");
+  const int64_t n0 = L->n[0], n1 = L->n[1], n2 = L->n[2];
+  const int h0 = L->halo[0], h1 = L->halo[1], h2 = L->halo[2];
+  ShellDesc S;
+  S.nA = 2 * h0 * n1 * n2;
+  S.nB = (n0 - 2 * h0) * 2 * h1 * n2;
+  S.nC = (n0 - 2 * h0) * (n1 - 2 * h1) * 2 * h2;
+  const int64_t total = S.nA + S.nB + S.nC;
+  if (total == 0) return 0;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  hipStream_t s = (hipStream_t)stream;
+  switch (L->elem_size) {
+    case 8: ghost_fill_kernel<uint64_t><<<blocks, 256, 0, s>>>((uint64_t*)d_f, *L, *V, S, all_pops); break;
+    case 4: ghost_fill_kernel<uint32_t><<<blocks, 256, 0, s>>>((uint32_t*)d_f, *L, *V, S, all_pops); break;
+    case 2: ghost_fill_kernel<uint16_t><<<blocks, 256, 0, s>>>((uint16_t*)d_f, *L, *V, S, all_pops); break;
+    default: LBM_REQUIRE(false, "bad element size %d", L->elem_size);
+  }
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int lbm_ghost_fill(void* d_f, const lbm_layout* layout, const lbm_velocities* vel, void* stream) {
+  return ghost_fill_impl(d_f, layout, vel, 0, stream);
+}
+
+int lbm_ghost_fill_all(void* d_f, const lbm_layout* layout, const lbm_velocities* vel,
+                       void* stream) {
+  return ghost_fill_impl(d_f, layout, vel, 1, stream);
+}
+
+int lbm_links_apply(void* d_f, const lbm_layout* layout, const uint32_t* d_store,
+                    const uint32_t* d_load, const void* d_rhs, const int8_t* d_sign, double sign,
+                    int64_t n, int two_phase, void* d_work, void* stream) {
+  int rc = check_layout(layout);
+  if (rc) return rc;
+  if (n <= 0) return 0;
+  LBM_REQUIRE(d_store && d_load, "store/load index arrays are NULL");
+  hipStream_t s = (hipStream_t)stream;
+  switch (layout->storage) {
+    case LBM_F64: return links_launch<double>(d_f, d_store, d_load, d_rhs, d_sign, sign, n, two_phase, d_work, s);
+    case LBM_F32: return links_launch<float>(d_f, d_store, d_load, d_rhs, d_sign, sign, n, two_phase, d_work, s);
+    case LBM_F16D: return links_launch<_Float16>(d_f, d_store, d_load, d_rhs, d_sign, sign, n, two_phase, d_work, s);
+    case LBM_BF16D: return links_launch<bf16>(d_f, d_store, d_load, d_rhs, d_sign, sign, n, two_phase, d_work, s);
+  }
+  LBM_REQUIRE(false, "unknown storage %d", layout->storage);
+}
+
+int lbm_reduce_blocks(const lbm_layout* layout, int64_t* nblocks) {
+  int rc = check_layout(layout);
+  if (rc) return rc;
+  const int64_t rows = interior(*layout, 0) * interior(*layout, 1);
+  *nblocks = rows < 1024 ? rows : 1024;
+  return 0;
+}
+
+int lbm_reduce_populations(const void* d_f, const lbm_layout* layout, const double* d_wshift,
+                           double* d_partial, double* h_sum, double* h_comp, void* stream) {
+  int64_t nb = 0;
+  int rc = lbm_reduce_blocks(layout, &nb);
+  if (rc) return rc;
+  LBM_REQUIRE(d_partial && h_sum && h_comp, "NULL output");
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)nb, (unsigned)layout->ns);
+  switch (layout->storage) {
+    case LBM_F64: reduce_pop_kernel<double><<<grid, kRedThreads, 0, s>>>((const double*)d_f, *layout, d_wshift, d_partial); break;
+    case LBM_F32: reduce_pop_kernel<float><<<grid, kRedThreads, 0, s>>>((const float*)d_f, *layout, d_wshift, d_partial); break;
+    case LBM_F16D: reduce_pop_kernel<_Float16><<<grid, kRedThreads, 0, s>>>((const _Float16*)d_f, *layout, d_wshift, d_partial); break;
+    case LBM_BF16D: reduce_pop_kernel<bf16><<<grid, kRedThreads, 0, s>>>((const bf16*)d_f, *layout, d_wshift, d_partial); break;
+    default: LBM_REQUIRE(false, "unknown storage");
+  }
+  LBM_CHECK_HIP(hipGetLastError());
+  // the final (sum, comp) pairs go to the tail of the partial buffer
+  double* out = d_partial + 2 * (int64_t)layout->ns * nb;
+  reduce_final_kernel<<<dim3((unsigned)layout->ns), kRedThreads, 0, s>>>(d_partial, nb, out);
+  LBM_CHECK_HIP(hipGetLastError());
+  double tmp[2 * LBM_MAX_Q];
+  LBM_CHECK_HIP(hipMemcpyAsync(tmp, out, sizeof(double) * 2 * layout->ns, hipMemcpyDeviceToHost, s));
+  LBM_CHECK_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < layout->ns; ++k) {
+    h_sum[k] = tmp[2 * k];
+    h_comp[k] = tmp[2 * k + 1];
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,624 @@
+"""HIP (gfx950) code generator for one lattice Boltzmann scheme.
+
+Plays the role of pylbm's code printers + ``CodeGen`` (pylbm/generator/codegen.py:
+1014-1178 for Cython, printing/cython.py:301-350) for the routines
+``one_time_step`` (pylbm/algorithm/pull.py:11-59), ``f2m``, ``m2f`` and
+``equilibrium`` (pylbm/algorithm/base.py:298-402): from the frozen
+:class:`~pylbm_amd.scheme.KernelSpec` it prints one ``.hip`` translation unit
+holding
+
+* ``fused_step<S, FOLD>``: pull f_k(x - v_k), m = M f, [source dt/2],
+  relax m_i <- (1 - s_i) m_i + s_i eq_i(m_c), [source dt/2], fnew = M^-1 m,
+  one thread per interior cell, stored through the population codec S;
+* ``f2m_kernel<S>`` / ``m2f_kernel<S>`` / ``equilibrium_kernel`` over the whole
+  halo-inclusive domain with FP64 moments;
+* the ``extern "C"`` entry points declared in include/pylbm_hip.h.
+
+The q x q transforms are printed from the exact rational matrices.  When every
+moment polynomial has a definite parity under v -> -v (true for the usual MRT
+bases) the transforms are factored through the pair sums f_k + f_ksym and
+differences f_k - f_ksym, which halves the work; terms sharing a coefficient
+are grouped and identical partial sums are computed once.  Relaxation rates
+are run-time arguments, so one compiled module serves every viscosity; only
+the zero pattern of s (which rows are relaxed at all, pylbm drops identity
+rows, printing/cython.py:347-349) is baked in.
+"""
+import hashlib
+from fractions import Fraction
+
+import sympy as sp
+from sympy.printing.c import C99CodePrinter
+
+from ..scheme import KernelSpec
+
+__all__ = ["generate_source", "spec_hash"]
+
+
+def spec_hash(spec: KernelSpec) -> str:
+    return hashlib.sha1(spec.key().encode()).hexdigest()[:16]
+
+
+def _lit(x) -> str:
+    """Correctly rounded FP64 literal of an exact rational, cast to the compute type."""
+    v = float(Fraction(x)) if not isinstance(x, float) else x
+    r = repr(v)
+    if "e" not in r and "." not in r and "inf" not in r and "nan" not in r:
+        r += ".0"
+    return "C(%s)" % r
+
+
+class _Printer(C99CodePrinter):
+    """Prints sympy expressions in the compute type ``C`` of the kernels."""
+
+    def _print_Symbol(self, expr):
+        name = expr.name
+        if name == "t":
+            return "tt"
+        return name
+
+    def _print_Integer(self, expr):
+        return _lit(int(expr))
+
+    def _print_Rational(self, expr):
+        return _lit(Fraction(int(expr.p), int(expr.q)))
+
+    def _print_Float(self, expr):
+        return _lit(float(expr))
+
+    def _print_Pow(self, expr):
+        b, e = expr.as_base_exp()
+        if e.is_Integer:
+            n = int(e)
+            if n == 0:
+                return "C(1.0)"
+            base = self.parenthesize(b, 100)
+            prod = "*".join([base] * abs(n))
+            return "(%s)" % prod if n > 0 else "(C(1.0)/(%s))" % prod
+        if e == sp.Rational(1, 2):
+            return "sqrt(%s)" % self._print(b)
+        raise NotImplementedError("power %s not supported" % expr)
+
+
+def _ccode(expr) -> str:
+    return _Printer().doprint(expr)
+
+
+# ----------------------------------------------------------------- structure
+def _canon(v, dim):
+    """Reference axes -> canonical (outer, mid, inner)."""
+    v = list(v)
+    return tuple([0] * (3 - dim) + v)
+
+
+def _pairs(spec: KernelSpec):
+    """ksym[k] inside the same block of M, or None if the stencil is not symmetric."""
+    ns = spec.ns
+    blocks = spec.blocks
+    ksym = [None] * ns
+    for lo, hi in blocks:
+        index = {tuple(spec.velocities[j]): j for j in range(lo, hi)}
+        for k in range(lo, hi):
+            neg = tuple(-c for c in spec.velocities[k])
+            if neg not in index:
+                return None
+            ksym[k] = index[neg]
+    return ksym
+
+
+def _row_parities(mat, ksym, by_rows=True):
+    """Parity sigma_i of every row i (by_rows: mat[i][ksym j] == sigma_i mat[i][j]) or
+    column i (mat[ksym j][i] == sigma_i mat[j][i]); None if one is indefinite."""
+    ns = len(mat)
+    get = (lambda i, j: mat[i][j]) if by_rows else (lambda i, j: mat[j][i])
+    sig = []
+    for i in range(ns):
+        s = 0
+        for j in range(ns):
+            a = get(i, j)
+            if a == 0 or ksym[j] == j:
+                continue
+            b = get(i, ksym[j])
+            if b == a:
+                s = 1
+            elif b == -a:
+                s = -1
+            else:
+                return None
+            break
+        s = s or 1
+        for j in range(ns):
+            if get(i, ksym[j]) != s * get(i, j):
+                return None
+        sig.append(s)
+    return sig
+
+
+class _SumBuilder:
+    """Emits grouped linear combinations, reusing identical partial sums."""
+
+    def __init__(self, prefix):
+        self.lines = []
+        self.cache = {}
+        self.prefix = prefix
+        self.count = 0
+
+    def _group(self, terms):
+        # terms: list of (sign, var) with sign +-1
+        key = tuple(sorted(terms))
+        if len(terms) == 1 and terms[0][0] == 1:
+            return terms[0][1]
+        if key in self.cache:
+            return self.cache[key]
+        pos = [v for s, v in terms if s > 0]
+        neg = [v for s, v in terms if s < 0]
+        if pos:
+            expr = " + ".join(pos) + "".join(" - %s" % v for v in neg)
+        else:
+            expr = "-(" + " + ".join(neg) + ")"
+        name = "%s%d" % (self.prefix, self.count)
+        self.count += 1
+        self.lines.append("const C %s = %s;" % (name, expr))
+        self.cache[key] = name
+        return name
+
+    def combo(self, coeffs):
+        """coeffs: list of (Fraction, var). Returns a C expression."""
+        groups = {}
+        for c, v in coeffs:
+            if c == 0:
+                continue
+            a = abs(c)
+            groups.setdefault(a, []).append((1 if c > 0 else -1, v))
+        if not groups:
+            return "C(0.0)"
+        parts = []
+        for a in sorted(groups):
+            g = self._group(groups[a])
+            parts.append(g if a == 1 else "%s*%s" % (_lit(a), g))
+        return " + ".join(parts)
+
+
+def _transform(builder, mat, invars, outvars, ksym, sig, rows_are_outputs=True):
+    """Lines computing outvars = mat @ invars (f2m) using pair parity if available."""
+    ns = len(mat)
+    lines = []
+    if ksym is not None and sig is not None:
+        # pair sums/differences of the inputs
+        reps = [k for k in range(ns) if k <= ksym[k]]
+        P, D = {}, {}
+        for k in reps:
+            if ksym[k] == k:
+                P[k] = invars[k]
+            else:
+                P[k] = builder._group([(1, invars[k]), (1, invars[ksym[k]])])
+                D[k] = builder._group([(1, invars[k]), (-1, invars[ksym[k]])])
+        for i in range(ns):
+            coeffs = []
+            for k in reps:
+                c = mat[i][k]
+                if c == 0:
+                    continue
+                if sig[i] > 0:
+                    coeffs.append((c, P[k]))
+                elif ksym[k] != k:
+                    coeffs.append((c, D[k]))
+            lines.append((outvars[i], builder.combo(coeffs)))
+    else:
+        for i in range(ns):
+            lines.append((outvars[i], builder.combo([(mat[i][j], invars[j]) for j in range(ns)])))
+    return lines
+
+
+def _inverse_transform(builder, inv, mvars, fvars, ksym, sig):
+    """Lines computing fvars = inv @ mvars; with parity f_k = E_k + O_k, f_ksym = E_k - O_k."""
+    ns = len(inv)
+    lines = []
+    if ksym is not None and sig is not None:
+        for k in range(ns):
+            if k > ksym[k]:
+                continue
+            even = [(inv[k][i], mvars[i]) for i in range(ns) if sig[i] > 0]
+            odd = [(inv[k][i], mvars[i]) for i in range(ns) if sig[i] < 0]
+            if ksym[k] == k:
+                lines.append((fvars[k], builder.combo(even)))
+                continue
+            e = "E%d" % k
+            o = "O%d" % k
+            builder.lines.append("const C %s = %s;" % (e, builder.combo(even)))
+            builder.lines.append("const C %s = %s;" % (o, builder.combo(odd)))
+            lines.append((fvars[k], "%s + %s" % (e, o)))
+            lines.append((fvars[ksym[k]], "%s - %s" % (e, o)))
+    else:
+        for k in range(ns):
+            lines.append((fvars[k], builder.combo([(inv[k][i], mvars[i]) for i in range(ns)])))
+    return lines
+
+
+def _collision_lines(spec: KernelSpec, with_sources: bool):
+    """Body statements after m0..m{ns-1} are available: sources, relax, sources."""
+    ns, nc = spec.ns, spec.nconsm
+    msyms = spec.msyms
+    out = []
+
+    def sources():
+        for idx, expr in spec.sources:
+            out.append("m%d = m%d + half_dt * (%s);" % (idx, idx, _ccode(sp.sympify(expr))))
+
+    if with_sources and spec.sources:
+        sources()
+    relaxed = [i for i in range(nc, ns) if spec.s[i] != 0]
+    eqs = [spec.eq[i] for i in relaxed]
+    repl, red = sp.cse(eqs, symbols=sp.numbered_symbols("q_"), optimizations="basic")
+    for sym, e in repl:
+        out.append("const C %s = %s;" % (sym, _ccode(e)))
+    for i, e in zip(relaxed, red):
+        out.append("m%d = one_minus[%d] * m%d + omega[%d] * (%s);" % (i, i, i, i, _ccode(e)))
+    if with_sources and spec.sources:
+        sources()
+    return out
+
+
+def _equilibrium_lines(spec: KernelSpec):
+    ns, nc = spec.ns, spec.nconsm
+    rows = list(range(nc, ns))
+    repl, red = sp.cse([spec.eq[i] for i in rows], symbols=sp.numbered_symbols("q_"),
+                       optimizations="basic")
+    out = ["const C %s = %s;" % (s, _ccode(e)) for s, e in repl]
+    for i, e in zip(rows, red):
+        out.append("m%d = %s;" % (i, _ccode(e)))
+    return out
+
+
+# ------------------------------------------------------------------ template
+_TEMPLATE = r"""// Generated by pylbm_amd.generator.hip for scheme %(hash)s -- do not edit.
+// ns=%(ns)d nconsm=%(nc)d dim=%(dim)d pair-factored=%(paired)s
+#include "lbm_device.h"
+
+using namespace lbm;
+
+#define NS %(ns)d
+#define NCONSM %(nc)d
+#define DIM %(dim)d
+
+namespace {
+
+template <typename C>
+struct StepArgs {
+  int32_t src_off[NS];   // pull source offset of population k relative to the cell
+  int32_t dst_off[NS];   // k * pop_stride
+  int32_t nin[3];        // interior extents
+  int32_t halo[3];
+  int32_t n1, pitch, lead, _pad;
+  C omega[NS], one_minus[NS], wshift[NS];
+  C half_dt, t;
+};
+
+template <typename S, bool FOLD>
+__global__ void __launch_bounds__(256)
+fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Codec<S>::C> a) {
+  using C = typename Codec<S>::C;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nin[2]) return;
+  const int mm = blockIdx.y, oo = blockIdx.z;
+  const int o = oo + a.halo[0], m = mm + a.halo[1], ii = i + a.halo[2];
+  const int cell = (o * a.n1 + m) * a.pitch + a.lead + ii;
+  const C half_dt = a.half_dt;
+  const C tt = a.t;
+  (void)half_dt; (void)tt;
+  const C* one_minus = a.one_minus;
+  const C* omega = a.omega;
+%(loads)s
+%(f2m)s
+%(coll)s
+%(m2f)s
+%(stores)s
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+f2m_kernel(const S* __restrict__ f, lbm_layout fl, double* __restrict__ mo, lbm_layout ml,
+           const StepArgs<double> a) {
+  using C = double;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= fl.n[2]) return;
+  const int64_t m_ = blockIdx.y, o_ = blockIdx.z;
+  const int64_t cf = cell_offset(fl, o_, m_, i);
+  const int64_t cm = cell_offset(ml, o_, m_, i);
+%(f2m_loads)s
+%(f2m_full)s
+%(f2m_stores)s
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+m2f_kernel(const double* __restrict__ mi, lbm_layout ml, S* __restrict__ f, lbm_layout fl,
+           const StepArgs<double> a) {
+  using C = double;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= fl.n[2]) return;
+  const int64_t m_ = blockIdx.y, o_ = blockIdx.z;
+  const int64_t cf = cell_offset(fl, o_, m_, i);
+  const int64_t cm = cell_offset(ml, o_, m_, i);
+%(m2f_loads)s
+%(m2f_full)s
+%(m2f_stores)s
+}
+
+__global__ void __launch_bounds__(256)
+equilibrium_kernel(double* __restrict__ mo, lbm_layout ml) {
+  using C = double;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= ml.n[2]) return;
+  const int64_t m_ = blockIdx.y, o_ = blockIdx.z;
+  const int64_t cm = cell_offset(ml, o_, m_, i);
+%(eq_loads)s
+%(eq_body)s
+%(eq_stores)s
+}
+
+template <typename C>
+static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
+  int rc = check_layout(L);
+  if (rc) return rc;
+  LBM_REQUIRE(L->ns == NS, "layout has ns=%%d, module was generated for %%d", L->ns, NS);
+  static const int hv[NS][3] = {%(vel)s};
+  for (int k = 0; k < NS; ++k) {
+    a->dst_off[k] = (int32_t)(k * L->pop_stride);
+    a->src_off[k] = (int32_t)(k * L->pop_stride - ((int64_t)hv[k][0] * L->n[1] + hv[k][1]) * L->pitch - hv[k][2]);
+  }
+  for (int d = 0; d < 3; ++d) {
+    a->nin[d] = (int32_t)(L->n[d] - 2 * L->halo[d]);
+    a->halo[d] = L->halo[d];
+  }
+  a->n1 = (int32_t)L->n[1];
+  a->pitch = (int32_t)L->pitch;
+  a->lead = (int32_t)L->lead;
+  a->_pad = 0;
+  for (int k = 0; k < NS; ++k) {
+    a->omega[k] = p ? (C)p->omega[k] : (C)0;
+    a->one_minus[k] = p ? (C)p->one_minus[k] : (C)1;
+    a->wshift[k] = p ? (C)p->wshift[k] : (C)0;
+  }
+  a->half_dt = p ? (C)(0.5 * p->dt) : (C)0;
+  a->t = p ? (C)p->t : (C)0;
+  return 0;
+}
+
+static unsigned block_x(int64_t n) {
+  return n >= 256 ? 256u : (n >= 128 ? 128u : 64u);
+}
+
+template <typename S>
+static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_params* p, int fold,
+                       hipStream_t s) {
+  using C = typename Codec<S>::C;
+  StepArgs<C> a;
+  int rc = make_args<C>(L, p, &a);
+  if (rc) return rc;
+  const unsigned bx = block_x(a.nin[2]);
+  dim3 grid((unsigned)((a.nin[2] + bx - 1) / bx), (unsigned)a.nin[1], (unsigned)a.nin[0]);
+  if (fold) {
+    for (int d = 0; d < 3; ++d)
+      LBM_REQUIRE(a.halo[d] <= 1 || a.nin[d] >= a.halo[d], "periodic fold needs extents >= halo");
+    fused_step<S, true><<<grid, bx, 0, s>>>((const S*)f, (S*)fn, a);
+  } else {
+    fused_step<S, false><<<grid, bx, 0, s>>>((const S*)f, (S*)fn, a);
+  }
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename S>
+static int f2m_launch(const void* f, const lbm_layout* fl, double* mo, const lbm_layout* ml,
+                      const lbm_params* p, hipStream_t s) {
+  StepArgs<double> a;
+  int rc = make_args<double>(fl, p, &a);
+  if (rc) return rc;
+  const unsigned bx = block_x(fl->n[2]);
+  dim3 grid((unsigned)((fl->n[2] + bx - 1) / bx), (unsigned)fl->n[1], (unsigned)fl->n[0]);
+  f2m_kernel<S><<<grid, bx, 0, s>>>((const S*)f, *fl, mo, *ml, a);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename S>
+static int m2f_launch(const double* mi, const lbm_layout* ml, void* f, const lbm_layout* fl,
+                      const lbm_params* p, hipStream_t s) {
+  StepArgs<double> a;
+  int rc = make_args<double>(fl, p, &a);
+  if (rc) return rc;
+  const unsigned bx = block_x(fl->n[2]);
+  dim3 grid((unsigned)((fl->n[2] + bx - 1) / bx), (unsigned)fl->n[1], (unsigned)fl->n[0]);
+  m2f_kernel<S><<<grid, bx, 0, s>>>(mi, *ml, (S*)f, *fl, a);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+static int same_extents(const lbm_layout* a, const lbm_layout* b) {
+  LBM_REQUIRE(a && b, "layout is NULL");
+  LBM_REQUIRE(b->storage == LBM_F64, "moment layout must be FP64");
+  for (int d = 0; d < 3; ++d)
+    LBM_REQUIRE(a->n[d] == b->n[d], "moment / population extents differ on axis %%d", d);
+  return check_layout(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_gen_info(int32_t* ns, int32_t* nconsm, int32_t* dim) {
+  *ns = NS;
+  *nconsm = NCONSM;
+  *dim = DIM;
+  return 0;
+}
+
+int lbm_gen_step(const void* d_f, void* d_fnew, const lbm_layout* layout, const lbm_params* params,
+                 int periodic_fold, void* stream) {
+  LBM_REQUIRE(layout, "layout is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  switch (layout->storage) {
+    case LBM_F64: return step_launch<double>(d_f, d_fnew, layout, params, periodic_fold, s);
+    case LBM_F32: return step_launch<float>(d_f, d_fnew, layout, params, periodic_fold, s);
+    case LBM_F16D: return step_launch<_Float16>(d_f, d_fnew, layout, params, periodic_fold, s);
+    case LBM_BF16D: return step_launch<bf16>(d_f, d_fnew, layout, params, periodic_fold, s);
+  }
+  LBM_REQUIRE(false, "unknown storage %%d", layout->storage);
+}
+
+int lbm_gen_f2m(const void* d_f, const lbm_layout* fl, double* d_m, const lbm_layout* ml,
+                const lbm_params* params, void* stream) {
+  int rc = same_extents(fl, ml);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  switch (fl->storage) {
+    case LBM_F64: return f2m_launch<double>(d_f, fl, d_m, ml, params, s);
+    case LBM_F32: return f2m_launch<float>(d_f, fl, d_m, ml, params, s);
+    case LBM_F16D: return f2m_launch<_Float16>(d_f, fl, d_m, ml, params, s);
+    case LBM_BF16D: return f2m_launch<bf16>(d_f, fl, d_m, ml, params, s);
+  }
+  LBM_REQUIRE(false, "unknown storage");
+}
+
+int lbm_gen_m2f(const double* d_m, const lbm_layout* ml, void* d_f, const lbm_layout* fl,
+                const lbm_params* params, void* stream) {
+  int rc = same_extents(fl, ml);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  switch (fl->storage) {
+    case LBM_F64: return m2f_launch<double>(d_m, ml, d_f, fl, params, s);
+    case LBM_F32: return m2f_launch<float>(d_m, ml, d_f, fl, params, s);
+    case LBM_F16D: return m2f_launch<_Float16>(d_m, ml, d_f, fl, params, s);
+    case LBM_BF16D: return m2f_launch<bf16>(d_m, ml, d_f, fl, params, s);
+  }
+  LBM_REQUIRE(false, "unknown storage");
+}
+
+int lbm_gen_equilibrium(double* d_m, const lbm_layout* ml, void* stream) {
+  int rc = check_layout(ml);
+  if (rc) return rc;
+  LBM_REQUIRE(ml->storage == LBM_F64 && ml->ns == NS, "equilibrium needs FP64 moments of ns=%%d", NS);
+  const unsigned bx = block_x(ml->n[2]);
+  dim3 grid((unsigned)((ml->n[2] + bx - 1) / bx), (unsigned)ml->n[1], (unsigned)ml->n[0]);
+  equilibrium_kernel<<<grid, bx, 0, (hipStream_t)stream>>>(d_m, *ml);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int lbm_gen_run(void* d_f, void* d_fnew, const lbm_layout* layout, const lbm_velocities* vel,
+                lbm_params* params, const lbm_link_set* links, int nmeth, int64_t nsteps,
+                int ghost, int periodic_fold, int* parity, void* stream) {
+  LBM_REQUIRE(params && layout && parity, "NULL argument");
+  LBM_REQUIRE(!(periodic_fold && nmeth > 0), "periodic fold is only valid without link lists");
+  void* a = d_f;
+  void* b = d_fnew;
+  for (int64_t it = 0; it < nsteps; ++it) {
+    int rc = 0;
+    if (ghost == 1) rc = lbm_ghost_fill(a, layout, vel, stream);
+    else if (ghost == 2) rc = lbm_ghost_fill_all(a, layout, vel, stream);
+    if (rc) return rc;
+    for (int j = 0; j < nmeth; ++j) {
+      const lbm_link_set& L = links[j];
+      rc = lbm_links_apply(a, layout, L.d_store, L.d_load, L.d_rhs, L.d_sign, L.sign, L.n,
+                           L.two_phase, L.d_work, stream);
+      if (rc) return rc;
+    }
+    rc = lbm_gen_step(a, b, layout, params, periodic_fold, stream);
+    if (rc) return rc;
+    void* tmp = a;
+    a = b;
+    b = tmp;
+    params->t += params->dt;
+  }
+  *parity = (int)(nsteps & 1);
+  return 0;
+}
+
+}  // extern "C"
+"""
+
+
+def _indent(lines, n=2):
+    return "\n".join(" " * n + ln for ln in lines)
+
+
+def generate_source(spec: KernelSpec) -> str:
+    ns, dim = spec.ns, spec.dim
+    vel = [_canon(v, dim) for v in spec.velocities]
+    ksym = _pairs(spec)
+    sig_f2m = _row_parities(spec.M, ksym, by_rows=True) if ksym else None
+    sig_m2f = None
+    if ksym and sig_f2m:
+        sig_cols = _row_parities(spec.invM, ksym, by_rows=False)
+        sig_m2f = sig_cols if sig_cols == sig_f2m else None
+        if sig_m2f is None:
+            sig_f2m = None
+    M = [[Fraction(c) for c in r] for r in spec.M]
+    iM = [[Fraction(c) for c in r] for r in spec.invM]
+    fvars = ["f%d" % k for k in range(ns)]
+    mvars = ["m%d" % i for i in range(ns)]
+
+    # ---------------------------------------------------------- fused step
+    loads = []
+    for k, (v0, v1, v2) in enumerate(vel):
+        loads.append("C f%d;" % k)
+    loads.append("if (!FOLD) {")
+    for k in range(ns):
+        loads.append("  f%d = Codec<S>::load(f + a.src_off[%d], (uint32_t)cell, a.wshift[%d]);" % (k, k, k))
+    loads.append("} else {")
+    # wrapped coordinates per distinct component value
+    comps = [sorted({v[d] for v in vel}) for d in range(3)]
+    names = {}
+    for d, axis in enumerate(("o", "m", "ii")):
+        for c in comps[d]:
+            nm = "w%d_%s" % (d, ("p%d" % c) if c >= 0 else ("n%d" % -c))
+            names[(d, c)] = nm
+            # source coordinate axis - c wrapped into the interior [h, h + nin)
+            loads.append("  int %s = %s - (%d);" % (nm, axis, c))
+            loads.append("  %s += (%s < a.halo[%d]) ? a.nin[%d] : 0;" % (nm, nm, d, d))
+            loads.append("  %s -= (%s >= a.halo[%d] + a.nin[%d]) ? a.nin[%d] : 0;" % (nm, nm, d, d, d))
+    for k, (v0, v1, v2) in enumerate(vel):
+        loads.append("  f%d = Codec<S>::load(f + a.dst_off[%d], (uint32_t)((%s * a.n1 + %s) * a.pitch + a.lead + %s), a.wshift[%d]);"
+                     % (k, k, names[(0, v0)], names[(1, v1)], names[(2, v2)], k))
+    loads.append("}")
+
+    b1 = _SumBuilder("p")
+    f2m_lines = _transform(b1, M, fvars, mvars, ksym, sig_f2m)
+    f2m_block = b1.lines + ["C %s = %s;" % (o, e) for o, e in f2m_lines]
+    coll = _collision_lines(spec, with_sources=True)
+    b2 = _SumBuilder("r")
+    m2f_lines = _inverse_transform(b2, iM, mvars, ["g%d" % k for k in range(ns)], ksym, sig_m2f)
+    m2f_block = b2.lines + ["const C %s = %s;" % (o, e) for o, e in m2f_lines]
+    stores = ["Codec<S>::store(fn + a.dst_off[%d], (uint32_t)cell, g%d, a.wshift[%d]);" % (k, k, k)
+              for k in range(ns)]
+
+    # ---------------------------------------------------------- f2m / m2f / eq
+    b3 = _SumBuilder("p")
+    f2m_full_lines = _transform(b3, M, fvars, mvars, ksym, sig_f2m)
+    f2m_loads = ["const C f%d = Codec<S>::widen(f[%d * fl.pop_stride + cf], (double)a.wshift[%d]);" % (k, k, k)
+                 for k in range(ns)]
+    f2m_full = b3.lines + ["const C %s = %s;" % (o, e) for o, e in f2m_full_lines]
+    f2m_stores = ["mo[%d * ml.pop_stride + cm] = m%d;" % (i, i) for i in range(ns)]
+
+    b4 = _SumBuilder("r")
+    m2f_full_lines = _inverse_transform(b4, iM, mvars, ["g%d" % k for k in range(ns)], ksym, sig_m2f)
+    m2f_loads = ["const C m%d = mi[%d * ml.pop_stride + cm];" % (i, i) for i in range(ns)]
+    m2f_full = b4.lines + ["const C %s = %s;" % (o, e) for o, e in m2f_full_lines]
+    m2f_stores = ["Codec<S>::store(f, %d * fl.pop_stride + cf, (typename Codec<S>::C)g%d, (typename Codec<S>::C)a.wshift[%d]);" % (k, k, k)
+                  for k in range(ns)]
+
+    eq_loads = ["C m%d = mo[%d * ml.pop_stride + cm];" % (i, i) for i in range(ns)]
+    eq_body = _equilibrium_lines(spec)
+    eq_stores = ["mo[%d * ml.pop_stride + cm] = m%d;" % (i, i) for i in range(spec.nconsm, ns)]
+    eq_loads += ["(void)m%d;" % i for i in range(spec.nconsm, ns)]
+
+    vel_txt = ", ".join("{%d, %d, %d}" % v for v in vel)
+    return _TEMPLATE % dict(
+        hash=spec_hash(spec), ns=ns, nc=spec.nconsm, dim=dim, paired=bool(sig_f2m),
+        vel=vel_txt,
+        loads=_indent(loads), f2m=_indent(f2m_block), coll=_indent(coll), m2f=_indent(m2f_block),
+        stores=_indent(stores),
+        f2m_loads=_indent(f2m_loads), f2m_full=_indent(f2m_full), f2m_stores=_indent(f2m_stores),
+        m2f_loads=_indent(m2f_loads), m2f_full=_indent(m2f_full), m2f_stores=_indent(m2f_stores),
+        eq_loads=_indent(eq_loads), eq_body=_indent(eq_body), eq_stores=_indent(eq_stores),
+    )

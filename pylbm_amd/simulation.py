@@ -1,0 +1,410 @@
+"""``Simulation(dico)`` / ``one_time_step()`` served by generated HIP kernels.
+
+Mirrors pylbm/simulation.py:89-420 for ``generator='hip'``:
+
+* construction: Domain, Scheme, code generation + compilation, containers,
+  boundary link lists, initialisation (simulation.py:89-163);
+* ``initialization``: init dict on the halo-inclusive meshgrid into the
+  conserved moments, then equilibrium + m2f on the GPU (simulation.py:257-320);
+* ``one_time_step``: periodic ghost fill, every link method in order, the
+  fused pull step, swap F / Fnew, t += dt, nt += 1 (simulation.py:373-420);
+* ``m[...]`` / ``F[...]`` read-back through a GPU f2m (simulation.py:200-243).
+
+Extensions that keep the reference semantics: ``run(nsteps)`` performs
+nsteps complete steps in one call into the library (no Python round-trip per
+step); ``dtype`` selects the population storage ('float64' FP64, 'float32',
+'float16' / 'bfloat16' = 16-bit deviations f_k - w_k rho0 with FP32
+arithmetic); ``conserved_sums()`` returns FP64 compensated global sums.
+"""
+import ctypes
+import math
+import time
+import types
+
+import numpy as np
+import sympy as sp
+
+from . import runtime as rtm
+from .boundary import Boundary, DeviceLinks
+from .domain import Domain
+from .generator.build import load_module
+from .monitoring import Monitor
+from .scheme import Scheme
+from .storage import STORAGE_BY_DTYPE, DeviceArray
+from .utils import itemproperty
+
+__all__ = ["Simulation"]
+
+LBM_F64 = rtm.LBM_F64
+
+
+def _canon_velocities(spec):
+    vel = rtm.LbmVelocities()
+    vel.q = spec.ns
+    for k, v in enumerate(spec.velocities):
+        c = [0] * (3 - spec.dim) + list(v)
+        for a in range(3):
+            vel.v[k][a] = c[a]
+    return vel
+
+
+class _MomentView:
+    """Moments of boundary points handed to user value functions (m[RHO] = ...)."""
+
+    def __init__(self, arr, consm):
+        self.arr = arr
+        self.consm = consm
+
+    def _k(self, key):
+        return key if isinstance(key, (int, np.integer)) else self.consm[key]
+
+    def __getitem__(self, key):
+        return self.arr[self._k(key)]
+
+    def __setitem__(self, key, value):
+        self.arr[self._k(key)] = value
+
+
+class _Container:
+    """F, Fnew and m of one simulation (pylbm/container.py:10-106)."""
+
+    gpu_support = True
+
+    def __init__(self, rt, ns, shape_halo, vmax, storage, consm, wshift):
+        self.F = DeviceArray(rt, ns, shape_halo, vmax, storage, wshift)
+        self.Fnew = DeviceArray(rt, ns, shape_halo, vmax, storage, wshift)
+        self._m = None
+        self._rt, self._shape, self._vmax, self._consm = rt, shape_halo, vmax, consm
+        for a in (self.F, self.Fnew):
+            a.set_conserved_moments(consm)
+        self.nv = ns
+        self.sorder = list(range(len(shape_halo) + 1))
+
+    @property
+    def m(self):
+        if self._m is None:
+            self._m = DeviceArray(self._rt, self.nv, self._shape, self._vmax, LBM_F64)
+            self._m.set_conserved_moments(self._consm)
+        return self._m
+
+
+class Simulation:
+    """Lattice Boltzmann simulation on one MI355X (pylbm.Simulation, generator 'hip')."""
+
+    def __init__(self, dico, sorder=None, dtype="float64", check_inverse=False, initialize=True,
+                 rho0=1.0, device=0):
+        gen = str(dico.get("generator", "hip")).upper()
+        if gen != "HIP":
+            raise ValueError("pylbm_amd provides the 'hip' generator only (got %r)" % dico.get("generator"))
+        if sorder is not None and list(sorder) != list(range(len(sorder))):
+            raise NotImplementedError("the hip generator uses its own device layout; sorder must be None")
+        if dtype not in STORAGE_BY_DTYPE:
+            raise ValueError("dtype must be one of %s" % sorted(STORAGE_BY_DTYPE))
+        self.rt = rtm.get(device)
+        self.domain = Domain(dico, need_validation=False)
+        Monitor.set_size(int(np.prod(self.domain.shape_in)))
+        self.scheme = Scheme(dico, check_inverse=check_inverse, need_validation=False)
+        if self.domain.dim != self.scheme.dim:
+            raise ValueError("the dimension of the domain and of the scheme are not the same")
+        self.dim = self.domain.dim
+        self._update_m = True
+        self.t = 0.0
+        self.nt = 0
+        self.dt_ = self.domain.dx / self.scheme.la
+        self.extra_parameters = {}
+        self.storage = STORAGE_BY_DTYPE[dtype]
+        self.dtype = dtype
+        self.rho0 = float(rho0)
+        self.spec = self.scheme.kernel_spec()
+        self.module = load_module(self.spec)
+        self.ns = self.spec.ns
+        self.vel = _canon_velocities(self.spec)
+        self.params = rtm.LbmParams()
+        for i, s in enumerate(self.spec.s):
+            sv = float(s)
+            self.params.omega[i] = sv
+            self.params.one_minus[i] = 1.0 - sv
+        self.params.dt = self.dt
+        vmax = list(self.domain.stencil.vmax)
+        wshift = self._rest_populations() if self.storage in (rtm.LBM_F16D, rtm.LBM_BF16D) else np.zeros(self.ns)
+        for k in range(self.ns):
+            self.params.wshift[k] = wshift[k]
+        self.wshift = wshift
+        self.container = _Container(self.rt, self.ns, self.domain.shape_halo, vmax, self.storage,
+                                    self.scheme.consm, wshift)
+        self.bc = Boundary(self.domain, dico)
+        for method in self.bc.methods:
+            method.set_iload()
+        self._dev_links = []
+        self._ghost_mode = 2 if any(m.kind.shift_axis is not None or not m.kind.symmetric_load
+                                    for m in self.bc.methods) else 1
+        self.init_type = dico.get("inittype", "moments")
+        self.init_data = dico.get("init", None)
+        self._need_init = True
+        if initialize:
+            self._initialize()
+
+    # ------------------------------------------------------------- basics
+    @property
+    def dt(self):
+        if isinstance(self.dt_, sp.Expr):
+            subs = list(self.scheme.param.items()) + list(self.extra_parameters.items())
+            self.dt_ = float(self.dt_.subs(subs))
+        return float(self.dt_)
+
+    def _check(self, rc, what):
+        self.rt.check(rc, what)
+
+    def _rest_populations(self):
+        """w_k rho0: m2f of the equilibrium with rho = rho0 and the other conserved moments 0."""
+        m = DeviceArray(self.rt, self.ns, [1] * self.spec.dim, [0] * self.spec.dim, LBM_F64)
+        host = np.zeros((self.ns,) + (1,) * self.spec.dim)
+        host[0] = self.rho0
+        m.set_fields(host)
+        f = DeviceArray(self.rt, self.ns, [1] * self.spec.dim, [0] * self.spec.dim, LBM_F64)
+        self._equilibrium(m)
+        self._m2f(m, f)
+        return f.fields().reshape(self.ns)
+
+    # ---------------------------------------------------------- kernels
+    def _equilibrium(self, m):
+        self._check(self.module.lbm_gen_equilibrium(ctypes.c_void_p(m.ptr), ctypes.byref(m.layout),
+                                                    self.rt.stream), "equilibrium")
+
+    def _m2f(self, m, f):
+        self._check(self.module.lbm_gen_m2f(ctypes.c_void_p(m.ptr), ctypes.byref(m.layout),
+                                            ctypes.c_void_p(f.ptr), ctypes.byref(f.layout),
+                                            ctypes.byref(self.params), self.rt.stream), "m2f")
+
+    def _f2m(self, f, m):
+        self._check(self.module.lbm_gen_f2m(ctypes.c_void_p(f.ptr), ctypes.byref(f.layout),
+                                            ctypes.c_void_p(m.ptr), ctypes.byref(m.layout),
+                                            ctypes.byref(self.params), self.rt.stream), "f2m")
+
+    def equilibrium(self, m_user=None):
+        self._equilibrium(m_user if m_user is not None else self.container.m)
+
+    def m2f(self, m_user=None, f_user=None):
+        self._m2f(m_user if m_user is not None else self.container.m,
+                  f_user if f_user is not None else self.container.F)
+
+    def f2m(self):
+        self._f2m(self.container.F, self.container.m)
+
+    # ---------------------------------------------------------- init
+    def _initialize(self):
+        self.initialization()
+        for method in self.bc.methods:
+            self._prepare_rhs(method)
+            method.fix_iload()
+            method.set_rhs()
+        self._dev_links = [DeviceLinks(self.rt, m, self.container.F.layout, self.wshift)
+                           for m in self.bc.methods]
+        self._link_sets = (rtm.LbmLinkSet * max(1, len(self._dev_links)))(
+            *[d.link_set() for d in self._dev_links])
+        self._need_init = False
+
+    def initialization(self):
+        coords = np.meshgrid(*self.domain.coords_halo, sparse=True, indexing="ij")
+        if self.init_type not in ("moments", "distributions"):
+            raise ValueError("inittype must be 'moments' or 'distributions'")
+        nc = self.scheme.nconsm if self.init_type == "moments" else self.ns
+        host = np.zeros((nc,) + tuple(self.domain.shape_halo))
+        view = _MomentView(host, self.scheme.consm)
+        for k, v in (self.init_data or {}).items():
+            if isinstance(v, tuple):
+                args = tuple(coords) + (tuple(v[1]) if len(v) == 2 else ())
+                view[k] = v[0](*args)
+            elif isinstance(v, types.FunctionType) or callable(v):
+                view[k] = v(*coords)
+            else:
+                view[k] = v
+        if self.init_type == "moments":
+            m = self.container.m
+            m.set_fields(host, 0)
+            self._equilibrium(m)
+            self._m2f(m, self.container.F)
+        else:
+            self.container.F.set_fields(host, 0)
+        self.container.Fnew.copy_from(self.container.F)
+        self.rt.sync()
+
+    def _prepare_rhs(self, method):
+        dx = self.domain.dx
+        v = self.scheme.stencil.get_all_velocities()
+        for key, value in method.value_bc.items():
+            if value is None:
+                continue
+            idx = np.where(method.ilabel == key)
+            n = idx[0].size
+            k = method.istore[0, idx]
+            s = 1 - method.distance[idx]
+            coords = ()
+            for i in range(self.dim):
+                x = self.domain.coords_halo[i][method.istore[i + 1, idx]]
+                x += s * v[k, i] * dx
+                x = x.ravel()
+                for _ in range(1, self.dim):
+                    x = x[:, np.newaxis]
+                coords += (x,)
+            func, args = (value[0], tuple(value[1])) if isinstance(value, tuple) else (value, ())
+            shape = [n] + [1] * (self.dim - 1)
+            host = np.zeros((self.ns,) + tuple(shape))
+            mv = _MomentView(host, self.scheme.consm)
+            if method.time_bc[key]:
+                func(None, mv, 0.0, *(coords + args))
+                method._tbc.append((key, func, coords + args, idx[0], shape))
+            else:
+                func(None, mv, *(coords + args))
+            method.feq[:, idx[0]] = self._feq(host, shape)
+
+    def _feq(self, host, shape):
+        m = DeviceArray(self.rt, self.ns, shape, [0] * len(shape), LBM_F64)
+        f = DeviceArray(self.rt, self.ns, shape, [0] * len(shape), LBM_F64)
+        m.set_fields(host)
+        self._equilibrium(m)
+        self._m2f(m, f)
+        return f.fields().reshape(self.ns, -1)
+
+    def _update_feq(self, method, dev):
+        """Time-dependent boundary values (pylbm/boundary.py:307-321)."""
+        if not method._tbc:
+            return
+        for key, func, args, cols, shape in method._tbc:
+            host = np.zeros((self.ns,) + tuple(shape))
+            func(None, _MomentView(host, self.scheme.consm), self.t, *args)
+            method.feq[:, cols] = self._feq(host, shape)
+        method.set_rhs()
+        dev.update_rhs(method.rhs, self.wshift)
+
+    # ---------------------------------------------------------- stepping
+    def boundary_condition(self):
+        f = self.container.F
+        fn = self.rt.lib.lbm_ghost_fill if self._ghost_mode == 1 else self.rt.lib.lbm_ghost_fill_all
+        self._check(fn(ctypes.c_void_p(f.ptr), ctypes.byref(f.layout), ctypes.byref(self.vel),
+                       self.rt.stream), "ghost fill")
+        for method, dev in zip(self.bc.methods, self._dev_links):
+            self._update_feq(method, dev)
+            dev.apply(f.ptr)
+
+    def _fused(self):
+        F, Fn = self.container.F, self.container.Fnew
+        self.params.t = self.t
+        self._check(self.module.lbm_gen_step(ctypes.c_void_p(F.ptr), ctypes.c_void_p(Fn.ptr),
+                                             ctypes.byref(F.layout), ctypes.byref(self.params), 0,
+                                             self.rt.stream), "one_time_step")
+
+    def one_time_step(self):
+        if self._need_init:
+            self._initialize()
+        t0 = time.perf_counter() if Monitor.enabled else 0.0
+        self._update_m = True
+        self.boundary_condition()
+        self._fused()
+        c = self.container
+        c.F, c.Fnew = c.Fnew, c.F
+        self.t += self.dt
+        self.nt += 1
+        if Monitor.enabled:
+            self.rt.sync()
+            Monitor.record("one_time_step", time.perf_counter() - t0)
+
+    @property
+    def periodic_only(self):
+        return not self.bc.methods and all(lab == -1 for lab in self.domain.box_label)
+
+    def run(self, nsteps, fold=None):
+        """nsteps calls of one_time_step in one library call (same results)."""
+        if self._need_init:
+            self._initialize()
+        if nsteps <= 0:
+            return
+        if any(m._tbc for m in self.bc.methods):
+            for _ in range(nsteps):
+                self.one_time_step()
+            return
+        self._update_m = True
+        fold = self.periodic_only if fold is None else fold
+        if fold and not self.periodic_only:
+            raise ValueError("periodic fold requires a fully periodic box without link lists")
+        c = self.container
+        self.params.t = self.t
+        parity = ctypes.c_int(0)
+        ghost = 0 if fold else self._ghost_mode
+        self._check(self.module.lbm_gen_run(ctypes.c_void_p(c.F.ptr), ctypes.c_void_p(c.Fnew.ptr),
+                                            ctypes.byref(c.F.layout), ctypes.byref(self.vel),
+                                            ctypes.byref(self.params), self._link_sets,
+                                            len(self._dev_links), int(nsteps), ghost, 1 if fold else 0,
+                                            ctypes.byref(parity), self.rt.stream), "run")
+        if parity.value:
+            c.F, c.Fnew = c.Fnew, c.F
+        self.nt += nsteps
+        self.t = self.t + nsteps * self.dt
+
+    def sync(self):
+        self.rt.sync()
+
+    # ---------------------------------------------------------- read-back
+    def _moments(self):
+        if self._update_m:
+            self._update_m = False
+            self.f2m()
+
+    @itemproperty
+    def m_halo(self, i):
+        self._moments()
+        return self.container.m[i]
+
+    @itemproperty
+    def m(self, i):
+        self._moments()
+        return self.container.m._in(i)
+
+    @itemproperty
+    def F_halo(self, i):
+        return self.container.F[i]
+
+    @itemproperty
+    def F(self, i):
+        return self.container.F._in(i)
+
+    def moments(self):
+        """All moments on the interior, FP64, shape (ns, *shape_in)."""
+        self._moments()
+        full = self.container.m.fields()
+        sl = (slice(None),) + tuple(slice(h, -h) if h > 0 else slice(None) for h in self.domain.stencil.vmax)
+        return full[sl]
+
+    def conserved_sums(self):
+        """Global sums of the conserved moments over the interior: FP64 compensated
+        per-population sums on the GPU, combined exactly with the rows of M."""
+        f = self.container.F
+        nb = ctypes.c_int64(0)
+        self._check(self.rt.lib.lbm_reduce_blocks(ctypes.byref(f.layout), ctypes.byref(nb)), "reduce")
+        part = self.rt.malloc(8 * 2 * self.ns * (nb.value + 1))
+        wptr = self.rt.upload(self.wshift.astype(np.float64)) if self.storage in (rtm.LBM_F16D, rtm.LBM_BF16D) else None
+        s = (ctypes.c_double * self.ns)()
+        c = (ctypes.c_double * self.ns)()
+        try:
+            self._check(self.rt.lib.lbm_reduce_populations(ctypes.c_void_p(f.ptr), ctypes.byref(f.layout),
+                                                           ctypes.c_void_p(wptr), ctypes.c_void_p(part),
+                                                           s, c, self.rt.stream), "reduce")
+        finally:
+            self.rt.free(part)
+            if wptr:
+                self.rt.free(wptr)
+        out = []
+        for i in range(self.scheme.nconsm):
+            terms = []
+            for k in range(self.ns):
+                coef = self.spec.M[i][k]
+                if coef != 0:
+                    terms += [float(coef) * s[k], float(coef) * c[k]]
+            out.append(math.fsum(terms))
+        return out
+
+    def __str__(self):
+        return ("Simulation(hip, dim=%d, shape=%s, ns=%d, storage=%s, t=%g, nt=%d, device=%s)"
+                % (self.dim, self.domain.shape_in, self.ns, self.dtype, self.t, self.nt, self.rt.device_name))
+
+    __repr__ = __str__
