@@ -1,0 +1,215 @@
+"""Benchmark: D3Q19 MRT Taylor-Green vortex, 256^3 cells, FP32 populations, one MI355X.
+
+A step is one ``Simulation.one_time_step`` pass of the hot path over the
+256^3 periodic box (for this fully periodic box the fused pull/collide kernel
+reads through the periodic wrap, so a step is exactly one kernel launch).
+The timed region starts with the populations resident in HBM: W warm-up steps,
+device sync, K timed steps between HIP events on the simulation's stream,
+device sync.  Printed as one JSON line:
+
+  value     MLUPS = cells * K / seconds / 1e6 (pylbm/monitoring.py:111)
+  roofline  algorithmic bytes of the fused kernel (19 loads + 19 stores of
+            4 B per cell = 152 B/cell) per launch / average launch time,
+            against the 8 TB/s HBM peak; traffic = HBM bytes per launch from
+            rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), scaled by the same
+            counters on a plain copy kernel of known size, or null
+  cpu_baseline  the oracle's C restatement of pylbm's generated Cython loop
+            (FP64, AoS, -O3) on a bounded 128^3 sample, all host threads
+
+``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS + % HBM roofline (D3Q19 256³ FP32); surrogate speedup at ≤1e-2 rel-L2"
+HBM_PEAK_GBPS = 8000.0
+BYTES = {"float32": 4, "float64": 8, "float16": 2, "bfloat16": 2}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fused_step_d3q19_256_fp32.txt")
+
+
+def _time_run(sim, steps, fold=None):
+    rt = sim.rt
+    e0, e1 = rt.event(), rt.event()
+    rt.sync()
+    t0 = time.perf_counter()
+    rt.record(e0)
+    sim.run(steps, fold=fold)
+    rt.record(e1)
+    ms = rt.elapsed_ms(e0, e1)
+    rt.sync()
+    wall = time.perf_counter() - t0
+    return ms, wall
+
+
+def _copy_bandwidth(rt, nbytes=2 << 30, reps=20):
+    """GB/s of the plain copy kernel (read + write) over nbytes buffers."""
+    src, dst = rt.malloc(nbytes), rt.malloc(nbytes)
+    try:
+        rt.check(rt.lib.lbm_copy(dst, src, nbytes, rt.stream), "lbm_copy")
+        e0, e1 = rt.event(), rt.event()
+        rt.record(e0)
+        for _ in range(reps):
+            rt.check(rt.lib.lbm_copy(dst, src, nbytes, rt.stream), "lbm_copy")
+        rt.record(e1)
+        ms = rt.elapsed_ms(e0, e1)
+    finally:
+        rt.free(src)
+        rt.free(dst)
+    return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
+
+
+def _traffic():
+    """HBM bytes per fused-kernel launch from the committed rocprofv3 PMC summary."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    txt = open(TRAFFIC_FILE).read()
+    m = re.search(r"^bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
+    return float(m.group(1)) if m else None
+
+
+def _cpu_baseline(target_s=8.0):
+    """MLUPS of the C restatement of the generated Cython loop on a 128^3 sample."""
+    from oracle.cgen import CStepper, set_threads
+    from oracle.scheme import Scheme as OScheme
+    from pylbm_amd import cases
+
+    n = 128
+    d = cases.taylor_green_d3q19(n)
+    sch = OScheme(d)
+    # initial state: equilibrium of the Taylor-Green fields (same init as the GPU run)
+    xs = np.arange(n + 2) - 0.5
+    X, Y, Z = np.meshgrid(xs, xs, xs, sparse=True, indexing="ij")
+    init = d["init"]
+    from pylbm_amd.cases import QX, QY, RHO
+    mc = [np.broadcast_to(init[RHO](X, Y, Z), (n + 2,) * 3), np.broadcast_to(init[QX](X, Y, Z), (n + 2,) * 3),
+          np.broadcast_to(init[QY](X, Y, Z), (n + 2,) * 3), np.zeros((n + 2,) * 3)]
+    eq = np.stack(sch.equilibrium(mc))
+    f = np.ascontiguousarray(np.moveaxis(np.tensordot(sch.invMnum, eq, axes=(1, 0)), 0, -1))
+    g = np.empty_like(f)
+    cs = CStepper(sch)
+    halo = (1, 1, 1)
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    cores = min(cores, 16)
+    out = {}
+    for threads in (1, cores):
+        set_threads(threads)
+        t0 = time.perf_counter()
+        cs.ghost(f, halo)
+        cs.step(f, g, halo, 0.0, 1.0)
+        one = time.perf_counter() - t0
+        steps = max(1, int(target_s / max(one, 1e-3)))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            cs.ghost(f, halo)
+            cs.step(f, g, halo, 0.0, 1.0)
+            f, g = g, f
+        sec = time.perf_counter() - t0
+        out[threads] = (n ** 3 * steps / sec / 1e6, steps)
+    return out, cores, n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="launch the plain copy kernel on a buffer of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from pylbm_amd import Simulation, cases
+
+    n = args.n
+    cells = n ** 3
+    sim = Simulation(cases.taylor_green_d3q19(n), dtype="float32")
+    rt = sim.rt
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        src, dst = rt.malloc(nb), rt.malloc(nb)
+        rt.check(rt.lib.lbm_copy(dst, src, nb, rt.stream), "lbm_copy")
+        rt.sync()
+        rt.free(src)
+        rt.free(dst)
+    sim.run(args.warmup)
+    ms, wall = _time_run(sim, args.steps)
+    sec = ms * 1e-3
+    mlups = cells * args.steps / sec / 1e6
+    alg_bytes = 2 * 19 * BYTES["float32"] * cells          # per launch
+    achieved = alg_bytes * args.steps / sec / 1e9           # GB/s
+    traffic = _traffic()
+
+    extra = {}
+    if not args.no_variants:
+        extra["copy_kernel_GBps"] = round(_copy_bandwidth(rt), 1)
+        ref64 = Simulation(cases.taylor_green_d3q19(n), dtype="float64")
+        ref64.run(args.warmup + args.steps)
+        u64 = [ref64.m[s] for s in (cases.QX, cases.QY, cases.QZ)]
+        scale = max(float(np.max(np.abs(u))) for u in u64)
+        errs = {"float32": max(float(np.max(np.abs(a - b))) for a, b in
+                               zip([sim.m[s] for s in (cases.QX, cases.QY, cases.QZ)], u64)) / scale}
+        vmlups = {}
+        for dt in ("float64", "float16", "bfloat16"):
+            v = ref64 if dt == "float64" else Simulation(cases.taylor_green_d3q19(n), dtype=dt)
+            if dt != "float64":
+                v.run(args.warmup + args.steps)
+                errs[dt] = max(float(np.max(np.abs(v.m[s] - u))) for s, u in
+                               zip((cases.QX, cases.QY, cases.QZ), u64)) / scale
+            vms, _ = _time_run(v, 200)
+            vmlups[dt] = round(cells * 200 / (vms * 1e-3) / 1e6, 1)
+            del v
+        hms, _ = _time_run(sim, 200, fold=False)
+        vmlups["float32_halo_ghost_fill"] = round(cells * 200 / (hms * 1e-3) / 1e6, 1)
+        extra["variant_MLUPS"] = vmlups
+        extra["velocity_err_vs_fp64_after_%d_steps" % (args.warmup + args.steps)] = {
+            k: float("%.3e" % v) for k, v in errs.items()}
+        del ref64
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        res, cores, cn = _cpu_baseline()
+        cpu = {"value": round(res[cores][0], 2), "unit": "MLUPS", "cores": cores, "kind": "port",
+               "value_1core": round(res[1][0], 2),
+               "sample": "%d^3 D3Q19 MRT Taylor-Green, FP64 AoS, C restatement of pylbm's generated Cython "
+                         "one_time_step + periodic ghost fill (oracle/cgen.py), %d steps on %d OpenMP threads, "
+                         "%d steps on 1 thread" % (cn, res[cores][1], cores, res[1][1])}
+
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "D3Q19 MRT periodic Taylor-Green vortex, %d^3 cells, u0=0.04, nu=0.005, "
+                               "FP32 populations, FP32 arithmetic" % n,
+                   "cells": cells, "storage": "float32", "device": rt.device_name,
+                   "wall_s_timed": round(wall, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "kernel": "fused_step<float,true> (pull + f2m + MRT relax + m2f)",
+                     "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -94,6 +94,10 @@ int lbm_event_destroy(void* event);
 int lbm_event_record(void* event, void* stream);
 int lbm_event_elapsed_ms(void* start, void* stop, float* ms);
 
+/* plain device copy kernel (16-byte vectors): bandwidth reference of the chip
+ * and the known-size kernel used to calibrate HBM byte counters */
+int lbm_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
+
 /*
  * Periodic ghost fill of every halo slot a pull step or a link list can read:
  * each ghost cell receives, for every population entering the interior through

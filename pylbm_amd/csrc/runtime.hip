@@ -245,6 +245,16 @@ __global__ void __launch_bounds__(kRedThreads) reduce_final_kernel(double* __res
   }
 }
 
+// ------------------------------------------------------------ plain copy
+// 16-byte vector copy, grid-stride; the bandwidth reference of the chip and the
+// known-size kernel that calibrates the HBM byte counters.
+__global__ void __launch_bounds__(256) copy_kernel(uint4* __restrict__ dst,
+                                                   const uint4* __restrict__ src, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -406,6 +416,19 @@ int lbm_links_apply(void* d_f, const lbm_layout* layout, const uint32_t* d_store
     case LBM_BF16D: return links_launch<bf16>(d_f, d_store, d_load, d_rhs, d_sign, sign, n, two_phase, d_work, s);
   }
   LBM_REQUIRE(false, "unknown storage %d", layout->storage);
+}
+
+int lbm_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream) {
+  LBM_REQUIRE(bytes % 16 == 0, "lbm_copy needs a multiple of 16 bytes");
+  LBM_REQUIRE(((uintptr_t)d_dst | (uintptr_t)d_src) % 16 == 0, "lbm_copy needs 16-byte alignment");
+  const int64_t n = (int64_t)(bytes / 16);
+  if (n == 0) return 0;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  copy_kernel<<<dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream>>>((uint4*)d_dst,
+                                                                           (const uint4*)d_src, n);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
 }
 
 int lbm_reduce_blocks(const lbm_layout* layout, int64_t* nblocks) {

@@ -72,6 +72,7 @@ CORE_SIGNATURES = {
     "lbm_memcpy_d2h": ([_P, _P, _U64, _P], _I),
     "lbm_memcpy_d2d": ([_P, _P, _U64, _P], _I),
     "lbm_memset": ([_P, _I, _U64, _P], _I),
+    "lbm_copy": ([_P, _P, _U64, _P], _I),
     "lbm_stream_create": ([ctypes.POINTER(_P)], _I),
     "lbm_stream_destroy": ([_P], _I),
     "lbm_stream_sync": ([_P], _I),
