@@ -1,0 +1,60 @@
+"""The C-ABI libraries load and export every entry point include/pylbm_hip.h
+declares (no GPU needed; nothing is called)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pylbm_hip.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(lbm_\w+)\s*\(", txt, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_parses():
+    names = _declared()
+    assert "lbm_gen_step" in names and "lbm_links_apply" in names and len(names) > 25
+
+
+def test_core_exports():
+    from pylbm_amd.runtime import CORE_SIGNATURES, load_core_library
+    lib = load_core_library()
+    core = [n for n in _declared() if not n.startswith("lbm_gen_")]
+    for n in core:
+        assert hasattr(lib, n), n
+    assert set(core) == set(CORE_SIGNATURES)
+
+
+def test_generated_exports():
+    from pylbm_amd import cases
+    from pylbm_amd.generator.build import module_path
+    from pylbm_amd.runtime import GEN_SIGNATURES, load_core_library
+    from pylbm_amd.scheme import Scheme
+    load_core_library()
+    path = module_path(Scheme(cases.taylor_green_d3q19(4)).kernel_spec())
+    assert os.path.exists(path), "run __graft_entry__.build() first"
+    lib = ctypes.CDLL(path)
+    gen = [n for n in _declared() if n.startswith("lbm_gen_")]
+    for n in gen:
+        assert hasattr(lib, n), n
+    assert set(gen) == set(GEN_SIGNATURES)
+
+
+def test_gpu_path_fails_loudly_without_device():
+    """Without a GPU the runtime raises instead of falling back to the CPU."""
+    import pylbm_amd.runtime as rt
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    with pytest.raises(rt.HipUnavailable):
+        rt.Runtime()

@@ -1,0 +1,103 @@
+"""Boundary link lists: product (pylbm_amd.boundary) == oracle (oracle.boundary),
+bit for bit in content and order, plus the invariants that pin the oracle
+(pylbm/boundary.py:23-160, 411-419, 777-890)."""
+import numpy as np
+import pytest
+
+from oracle.boundary import build_links
+from oracle.domain import Domain as ODomain
+from oracle.lbm import method_kind
+from pylbm_amd import cases
+from pylbm_amd.boundary import Boundary, _resolve_sequential
+from pylbm_amd.domain import Domain
+
+CASES = {
+    "lid": lambda: cases.lid_cavity_d2q9(16),
+    "karman": lambda: cases.karman_d2q9(64, 32, 5.0, (16.0, 16.0)),
+    "spheres": lambda: cases.sphere_packing_d3q19(20, 14, 12, seed=3, solid_fraction=0.15),
+    "d1q3": lambda: cases.d1q3_advection(12, walls=True),
+    "vectorial": lambda: cases.d2q5q9_vectorial(12, walls=True),
+}
+
+
+def _product(d):
+    dom = Domain(d)
+    b = Boundary(dom, d)
+    for m in b.methods:
+        m.set_iload()
+    return dom, b.methods
+
+
+def _oracle(d):
+    dom = ODomain(d)
+    from oracle.scheme import Scheme
+    st = Scheme(d).stencil
+    return dom, build_links(dom, st, d["boundary_conditions"], method_kind)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_links_identical(name):
+    d = CASES[name]()
+    _, pm = _product(d)
+    _, om = _oracle(d)
+    assert [m.name for m in pm] == [m.kind for m in om]
+    for a, b in zip(pm, om):
+        assert np.array_equal(a.istore, b.istore)
+        assert np.array_equal(a.iload[0], b.iload)
+        assert np.array_equal(a.distance, b.distance)
+        assert np.array_equal(a.ilabel, b.ilabel)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_link_invariants(name):
+    d = CASES[name]()
+    dom, om = _oracle(d)
+    from oracle.scheme import Scheme
+    st = Scheme(d).stencil
+    v = st.all_velocities()
+    fluid = dom.in_or_out == 999
+    total = 0
+    for m in om:
+        k = m.istore[0]
+        x = m.istore[1:] + v[k].T
+        assert fluid[tuple(x)].all()                           # x_out + v_k is fluid
+        total += k.size
+        # store slots are unique
+        flat = np.ravel_multi_index(tuple(m.istore), (st.nv_ptr[-1],) + tuple(dom.shape_halo))
+        assert np.unique(flat).size == flat.size
+    # brute-force count: (fluid x, k) with x - v_k outside the fluid and not periodic
+    per = [lab == -1 for lab in dom.box_label]
+    count = 0
+    shape = dom.shape_halo
+    for k, vk in enumerate(v):
+        for x in zip(*np.nonzero(fluid)):
+            y = [x[a] - vk[a] for a in range(dom.dim)]
+            wrapped = False
+            for a in range(dom.dim):
+                h = st.vmax[a]
+                if y[a] < h or y[a] >= shape[a] - h:
+                    if per[2 * a]:
+                        wrapped = True
+            if not wrapped and not fluid[tuple(y)]:
+                count += 1
+    assert total == count
+
+
+def test_resolve_sequential_matches_loop():
+    rng = np.random.default_rng(0)
+    for trial in range(50):
+        n, size = 30, 40
+        st = rng.integers(0, size, n)
+        ld = rng.integers(0, size, n)
+        sign = rng.choice([-1.0, 1.0], n)
+        rhs = rng.random(n)
+        f = rng.random(size)
+        seq = f.copy()
+        for i in range(n):
+            seq[st[i]] = sign[i] * seq[ld[i]] + rhs[i]
+        s2, l2, a2, r2 = _resolve_sequential(st, ld, sign, rhs)
+        par = f.copy()
+        gathered = a2 * f[l2] + r2
+        par[s2] = gathered
+        assert np.allclose(par, seq, rtol=0, atol=1e-12)
+        assert np.unique(s2).size == s2.size
