@@ -34,12 +34,16 @@
 extern "C" {
 #endif
 
-/* population storage formats (the compute type is FP64 for LBM_F64, FP32 otherwise) */
+/* population storage formats.  LBM_F64 computes in FP64 with full values, as
+ * pylbm does.  Every other format computes in FP32 on deviations from the rest
+ * state (f_k - w_k rho0, moments m - m_rest), which is the same affine
+ * collision in exact arithmetic and avoids the cancellation of O(rho) terms. */
 enum lbm_storage {
   LBM_F64 = 0,   /* double                                                   */
   LBM_F32 = 1,   /* float                                                    */
   LBM_F16D = 2,  /* IEEE half of the deviation f_k - w_k rho0               */
-  LBM_BF16D = 3  /* bfloat16 of the deviation f_k - w_k rho0                */
+  LBM_BF16D = 3, /* bfloat16 of the deviation f_k - w_k rho0                */
+  LBM_F32D = 4   /* float of the deviation f_k - w_k rho0                   */
 };
 
 #define LBM_MAX_Q 32
@@ -142,12 +146,14 @@ int lbm_reduce_populations(const void* d_f, const lbm_layout* layout, const doub
 /*
  * Run-time parameters of the generated kernels.  omega[i] = s_i and
  * one_minus[i] = 1 - s_i for every relaxed moment (pylbm/algorithm/base.py:404-428);
- * wshift[k] = w_k rho0 is the rest population removed by the deviation formats.
+ * wshift[k] = w_k rho0 = (M^-1 eq(cref))_k is the rest population that the
+ * FP32-compute formats subtract before the collision.
  */
 typedef struct lbm_params {
   double omega[LBM_MAX_Q];
   double one_minus[LBM_MAX_Q];
   double wshift[LBM_MAX_Q];
+  double cref[LBM_MAX_Q];   /* reference conserved moments (rest state) of the deviation formats */
   double dt;
   double t;
 } lbm_params;

@@ -27,7 +27,7 @@ import ctypes
 import numpy as np
 
 from . import bc as bcmod
-from .runtime import LBM_F16D, LBM_BF16D, LBM_F64, LbmLinkSet
+from .runtime import DEVIATION_FORMATS, LBM_F64, LbmLinkSet
 
 __all__ = ["Boundary", "BoundaryMethod"]
 
@@ -185,7 +185,7 @@ class DeviceLinks:
             self.n = st.size
         kstore = (st // layout.pop_stride).astype(np.int64)
         kload = (ld // layout.pop_stride).astype(np.int64)
-        if layout.storage in (LBM_F16D, LBM_BF16D):
+        if layout.storage in DEVIATION_FORMATS:
             # deviations: d_s = a d_l + (r + a w_l - w_s)
             rhs = rhs + sign * wshift[kload] - wshift[kstore]
         self.rhs_host = rhs
@@ -210,7 +210,7 @@ class DeviceLinks:
         """Re-upload rhs (time-dependent boundary values)."""
         if self.conflict:
             raise NotImplementedError("time-dependent values on a self-overlapping link list")
-        if self.layout.storage in (LBM_F16D, LBM_BF16D):
+        if self.layout.storage in DEVIATION_FORMATS:
             st = _flat(self.method._store_rows(), self.layout)
             ld = _flat(self.method.iload[0].T if self.method._fixed else self.method.iload[0], self.layout)
             rhs = rhs + self.method.kind.sign * wshift[ld // self.layout.pop_stride] - wshift[st // self.layout.pop_stride]

@@ -50,10 +50,16 @@ __device__ __forceinline__ bf16 float_to_bf16(float x) {
   return r;
 }
 
+// FP32 deviation storage (distinct type so it gets its own codec)
+struct f32dev {
+  float x;
+};
+
 // ------------------------------------------------------------ storage codec
-// Codec<S>::C is the compute type, load/store convert a stored population
-// to/from it; w is the rest population w_k rho0 removed by the deviation
-// formats (ignored by the full-value formats).
+// Codec<S>::C is the compute type.  load/store convert a stored population
+// to/from its full value; load_dev/store_dev to/from its deviation from the
+// rest population w (= w_k rho0): the deviation formats store it directly, the
+// FP32 full-value format subtracts / adds w (exact for w/2 <= f <= 2w).
 template <typename S>
 struct Codec;
 
@@ -63,6 +69,8 @@ struct Codec<double> {
   static constexpr int kind = LBM_F64;
   __device__ __forceinline__ static C load(const double* p, int64_t i, C) { return p[i]; }
   __device__ __forceinline__ static void store(double* p, int64_t i, C v, C) { p[i] = v; }
+  __device__ __forceinline__ static C load_dev(const double* p, int64_t i, C w) { return p[i] - w; }
+  __device__ __forceinline__ static void store_dev(double* p, int64_t i, C d, C w) { p[i] = w + d; }
   __device__ __forceinline__ static double widen(double s, double) { return s; }
 };
 
@@ -72,7 +80,20 @@ struct Codec<float> {
   static constexpr int kind = LBM_F32;
   __device__ __forceinline__ static C load(const float* p, int64_t i, C) { return p[i]; }
   __device__ __forceinline__ static void store(float* p, int64_t i, C v, C) { p[i] = v; }
+  __device__ __forceinline__ static C load_dev(const float* p, int64_t i, C w) { return p[i] - w; }
+  __device__ __forceinline__ static void store_dev(float* p, int64_t i, C d, C w) { p[i] = w + d; }
   __device__ __forceinline__ static double widen(float s, double) { return (double)s; }
+};
+
+template <>
+struct Codec<f32dev> {
+  using C = float;
+  static constexpr int kind = LBM_F32D;
+  __device__ __forceinline__ static C load(const f32dev* p, int64_t i, C w) { return w + p[i].x; }
+  __device__ __forceinline__ static void store(f32dev* p, int64_t i, C v, C w) { p[i].x = v - w; }
+  __device__ __forceinline__ static C load_dev(const f32dev* p, int64_t i, C) { return p[i].x; }
+  __device__ __forceinline__ static void store_dev(f32dev* p, int64_t i, C d, C) { p[i].x = d; }
+  __device__ __forceinline__ static double widen(f32dev s, double w) { return w + (double)s.x; }
 };
 
 template <>
@@ -85,6 +106,8 @@ struct Codec<_Float16> {
   __device__ __forceinline__ static void store(_Float16* p, int64_t i, C v, C w) {
     p[i] = (_Float16)(v - w);
   }
+  __device__ __forceinline__ static C load_dev(const _Float16* p, int64_t i, C) { return (float)p[i]; }
+  __device__ __forceinline__ static void store_dev(_Float16* p, int64_t i, C d, C) { p[i] = (_Float16)d; }
   __device__ __forceinline__ static double widen(_Float16 s, double w) { return w + (double)(float)s; }
 };
 
@@ -98,6 +121,8 @@ struct Codec<bf16> {
   __device__ __forceinline__ static void store(bf16* p, int64_t i, C v, C w) {
     p[i] = float_to_bf16(v - w);
   }
+  __device__ __forceinline__ static C load_dev(const bf16* p, int64_t i, C) { return bf16_to_float(p[i]); }
+  __device__ __forceinline__ static void store_dev(bf16* p, int64_t i, C d, C) { p[i] = float_to_bf16(d); }
   __device__ __forceinline__ static double widen(bf16 s, double w) {
     return w + (double)bf16_to_float(s);
   }
@@ -109,9 +134,28 @@ inline int storage_elem_size(int storage) {
     case LBM_F32: return 4;
     case LBM_F16D: return 2;
     case LBM_BF16D: return 2;
+    case LBM_F32D: return 4;
     default: return 0;
   }
 }
+
+inline bool is_deviation_storage(int storage) {
+  return storage == LBM_F16D || storage == LBM_BF16D || storage == LBM_F32D;
+}
+
+// Calls FN<S>(args...) with S the storage type of `kind`; returns -1 on an unknown kind.
+#define LBM_DISPATCH_STORAGE(kind, FN, ...)                                   \
+  do {                                                                        \
+    switch (kind) {                                                           \
+      case LBM_F64: return FN<double>(__VA_ARGS__);                           \
+      case LBM_F32: return FN<float>(__VA_ARGS__);                            \
+      case LBM_F16D: return FN<_Float16>(__VA_ARGS__);                        \
+      case LBM_BF16D: return FN<::lbm::bf16>(__VA_ARGS__);                    \
+      case LBM_F32D: return FN<::lbm::f32dev>(__VA_ARGS__);                   \
+    }                                                                         \
+    ::lbm::set_error("unknown storage kind %d", (int)(kind));                 \
+    return -1;                                                                \
+  } while (0)
 
 // interior extents of the canonical axes
 __host__ __device__ __forceinline__ int64_t interior(const lbm_layout& L, int a) {

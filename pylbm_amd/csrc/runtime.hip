@@ -255,6 +255,18 @@ __global__ void __launch_bounds__(256) copy_kernel(uint4* __restrict__ dst,
     dst[i] = src[i];
 }
 
+template <typename S>
+static int reduce_launch(const void* d_f, const lbm_layout* L, const double* wshift, double* partial,
+                         dim3 grid, hipStream_t s) {
+  reduce_pop_kernel<S><<<grid, kRedThreads, 0, s>>>((const S*)d_f, *L, wshift, partial);
+  return 0;
+}
+
+static int reduce_launch_any(int storage, const void* d_f, const lbm_layout* L, const double* wshift,
+                             double* partial, dim3 grid, hipStream_t s) {
+  LBM_DISPATCH_STORAGE(storage, reduce_launch, d_f, L, wshift, partial, grid, s);
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -409,13 +421,8 @@ int lbm_links_apply(void* d_f, const lbm_layout* layout, const uint32_t* d_store
   if (n <= 0) return 0;
   LBM_REQUIRE(d_store && d_load, "store/load index arrays are NULL");
   hipStream_t s = (hipStream_t)stream;
-  switch (layout->storage) {
-    case LBM_F64: return links_launch<double>(d_f, d_store, d_load, d_rhs, d_sign, sign, n, two_phase, d_work, s);
-    case LBM_F32: return links_launch<float>(d_f, d_store, d_load, d_rhs, d_sign, sign, n, two_phase, d_work, s);
-    case LBM_F16D: return links_launch<_Float16>(d_f, d_store, d_load, d_rhs, d_sign, sign, n, two_phase, d_work, s);
-    case LBM_BF16D: return links_launch<bf16>(d_f, d_store, d_load, d_rhs, d_sign, sign, n, two_phase, d_work, s);
-  }
-  LBM_REQUIRE(false, "unknown storage %d", layout->storage);
+  LBM_DISPATCH_STORAGE(layout->storage, links_launch, d_f, d_store, d_load, d_rhs, d_sign, sign, n,
+                       two_phase, d_work, s);
 }
 
 int lbm_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream) {
@@ -447,13 +454,8 @@ int lbm_reduce_populations(const void* d_f, const lbm_layout* layout, const doub
   LBM_REQUIRE(d_partial && h_sum && h_comp, "NULL output");
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nb, (unsigned)layout->ns);
-  switch (layout->storage) {
-    case LBM_F64: reduce_pop_kernel<double><<<grid, kRedThreads, 0, s>>>((const double*)d_f, *layout, d_wshift, d_partial); break;
-    case LBM_F32: reduce_pop_kernel<float><<<grid, kRedThreads, 0, s>>>((const float*)d_f, *layout, d_wshift, d_partial); break;
-    case LBM_F16D: reduce_pop_kernel<_Float16><<<grid, kRedThreads, 0, s>>>((const _Float16*)d_f, *layout, d_wshift, d_partial); break;
-    case LBM_BF16D: reduce_pop_kernel<bf16><<<grid, kRedThreads, 0, s>>>((const bf16*)d_f, *layout, d_wshift, d_partial); break;
-    default: LBM_REQUIRE(false, "unknown storage");
-  }
+  rc = reduce_launch_any(layout->storage, d_f, layout, d_wshift, d_partial, grid, s);
+  if (rc) return rc;
   LBM_CHECK_HIP(hipGetLastError());
   // the final (sum, comp) pairs go to the tail of the partial buffer
   double* out = d_partial + 2 * (int64_t)layout->ns * nb;

@@ -54,6 +54,8 @@ class _Printer(C99CodePrinter):
         name = expr.name
         if name == "t":
             return "tt"
+        if name.startswith("cr") and name[2:].isdigit():
+            return "cref[%s]" % name[2:]
         return name
 
     def _print_Integer(self, expr):
@@ -209,35 +211,49 @@ def _transform(builder, mat, invars, outvars, ksym, sig, rows_are_outputs=True):
     return lines
 
 
-def _inverse_transform(builder, inv, mvars, fvars, ksym, sig):
-    """Lines computing fvars = inv @ mvars; with parity f_k = E_k + O_k, f_ksym = E_k - O_k."""
+def _inverse_transform(builder, inv, mvars, fvars, ksym, sig, base=None, tag=""):
+    """Lines computing fvars = [base +] inv @ mvars (entries of mvars that are None
+    are zero); with parity f_k = E_k + O_k, f_ksym = E_k - O_k."""
     ns = len(inv)
     lines = []
+
+    def plus(k, expr):
+        return "%s + (%s)" % (base[k], expr) if base is not None else expr
+
     if ksym is not None and sig is not None:
         for k in range(ns):
             if k > ksym[k]:
                 continue
-            even = [(inv[k][i], mvars[i]) for i in range(ns) if sig[i] > 0]
-            odd = [(inv[k][i], mvars[i]) for i in range(ns) if sig[i] < 0]
+            even = [(inv[k][i], mvars[i]) for i in range(ns) if sig[i] > 0 and mvars[i] is not None]
+            odd = [(inv[k][i], mvars[i]) for i in range(ns) if sig[i] < 0 and mvars[i] is not None]
             if ksym[k] == k:
-                lines.append((fvars[k], builder.combo(even)))
+                lines.append((fvars[k], plus(k, builder.combo(even))))
                 continue
-            e = "E%d" % k
-            o = "O%d" % k
+            e = "E%s%d" % (tag, k)
+            o = "O%s%d" % (tag, k)
             builder.lines.append("const C %s = %s;" % (e, builder.combo(even)))
             builder.lines.append("const C %s = %s;" % (o, builder.combo(odd)))
-            lines.append((fvars[k], "%s + %s" % (e, o)))
-            lines.append((fvars[ksym[k]], "%s - %s" % (e, o)))
+            lines.append((fvars[k], plus(k, "%s + %s" % (e, o))))
+            lines.append((fvars[ksym[k]], plus(ksym[k], "%s - %s" % (e, o))))
     else:
         for k in range(ns):
-            lines.append((fvars[k], builder.combo([(inv[k][i], mvars[i]) for i in range(ns)])))
+            lines.append((fvars[k], plus(k, builder.combo([(inv[k][i], mvars[i]) for i in range(ns)
+                                                           if mvars[i] is not None]))))
     return lines
 
 
+def _deviation_eq(expr, full, rest):
+    """eq(c_ref + m') - eq(c_ref) with the constant cancelled symbolically."""
+    e = sp.expand(sp.sympify(expr).xreplace(full) - sp.sympify(expr).xreplace(rest))
+    zero = {s: 0 for s in full}
+    if sp.simplify(e.xreplace(zero)) != 0:
+        e = sp.cancel(sp.together(e))
+    return e
+
+
 def _collision_lines(spec: KernelSpec, with_sources: bool):
-    """Body statements after m0..m{ns-1} are available: sources, relax, sources."""
+    """Full-value collision (FP64, as pylbm): sources, relax, sources on m0..m{ns-1}."""
     ns, nc = spec.ns, spec.nconsm
-    msyms = spec.msyms
     out = []
 
     def sources():
@@ -247,8 +263,8 @@ def _collision_lines(spec: KernelSpec, with_sources: bool):
     if with_sources and spec.sources:
         sources()
     relaxed = [i for i in range(nc, ns) if spec.s[i] != 0]
-    eqs = [spec.eq[i] for i in relaxed]
-    repl, red = sp.cse(eqs, symbols=sp.numbered_symbols("q_"), optimizations="basic")
+    repl, red = sp.cse([spec.eq[i] for i in relaxed], symbols=sp.numbered_symbols("q_"),
+                       optimizations="basic")
     for sym, e in repl:
         out.append("const C %s = %s;" % (sym, _ccode(e)))
     for i, e in zip(relaxed, red):
@@ -256,6 +272,56 @@ def _collision_lines(spec: KernelSpec, with_sources: bool):
     if with_sources and spec.sources:
         sources()
     return out
+
+
+def _deviation_collision_lines(spec: KernelSpec):
+    """Collision of the FP32-compute formats, in increments.
+
+    The m_i are moments of d = f - w (w the rest populations of the reference
+    state c_ref = (rho0, 0, ..., 0), rho0 = cref[0] at run time), so the
+    conserved moments are c_ref + m_c and the target of a relaxed row is
+    eq_i(c_ref + m_c) - eq_i(c_ref).  The kernel computes the change of every
+    moment, dm_i = s_i (eq_i' - m_i) for relaxed rows and the source half
+    steps for source rows, and applies fnew = d + M^-1 dm: equal in exact
+    arithmetic to M^-1 (m + dm), but the rounding of the M^-1 coefficients
+    now multiplies the small non-equilibrium change instead of the moments,
+    and the rest state (dm = 0) is reproduced exactly.
+
+    Returns (lines, names) with names[i] the C name of dm_i or None."""
+    ns, nc = spec.ns, spec.nconsm
+    msyms = spec.msyms
+    crefs = [sp.Symbol("cr0")] + [sp.Integer(0)] * (nc - 1) if nc else []
+    full = {msyms[i]: crefs[i] + msyms[i] for i in range(nc)}
+    rest = {msyms[i]: crefs[i] for i in range(nc)}
+    out = []
+    names = [None] * ns
+    src_rows = sorted({idx for idx, _ in spec.sources})
+    for idx in src_rows:
+        out.append("C dm%d = C(0.0);" % idx)
+        names[idx] = "dm%d" % idx
+
+    def sources():
+        for idx, expr in spec.sources:
+            e = sp.sympify(expr).xreplace(full)
+            out.append("{ const C inc = half_dt * (%s); m%d += inc; dm%d += inc; }" % (_ccode(e), idx, idx))
+
+    sources()
+    relaxed = [i for i in range(nc, ns) if spec.s[i] != 0]
+    eqs = [_deviation_eq(spec.eq[i], full, rest) for i in relaxed]
+    repl, red = sp.cse(eqs, symbols=sp.numbered_symbols("q_"), optimizations="basic")
+    for sym, e in repl:
+        out.append("const C %s = %s;" % (sym, _ccode(e)))
+    for i, e in zip(relaxed, red):
+        target = "C(0.0)" if e == 0 else "(%s)" % _ccode(e)
+        if names[i] is None:
+            out.append("C dm%d = omega[%d] * (%s - m%d);" % (i, i, target, i))
+            names[i] = "dm%d" % i
+        else:
+            out.append("{ const C inc = omega[%d] * (%s - m%d); dm%d += inc; m%d += inc; }" % (i, target, i, i, i))
+            continue
+        out.append("m%d += dm%d;" % (i, i))
+    sources()
+    return out, names
 
 
 def _equilibrium_lines(spec: KernelSpec):
@@ -278,6 +344,7 @@ using namespace lbm;
 
 #define NS %(ns)d
 #define NCONSM %(nc)d
+#define NCREF (NCONSM > 0 ? NCONSM : 1)
 #define DIM %(dim)d
 
 namespace {
@@ -290,6 +357,7 @@ struct StepArgs {
   int32_t halo[3];
   int32_t n1, pitch, lead, _pad;
   C omega[NS], one_minus[NS], wshift[NS];
+  C cref[NCREF];
   C half_dt, t;
 };
 
@@ -307,11 +375,21 @@ fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename 
   (void)half_dt; (void)tt;
   const C* one_minus = a.one_minus;
   const C* omega = a.omega;
+  const C* cref = a.cref;
+  (void)cref;
+  // full-value arithmetic for FP64 (as pylbm), deviations from the rest state otherwise
+  constexpr bool DEV = Codec<S>::kind != LBM_F64;
 %(loads)s
 %(f2m)s
+  if constexpr (DEV) {
+%(coll_dev)s
+%(m2f_dev)s
+%(stores_dev)s
+  } else {
 %(coll)s
 %(m2f)s
 %(stores)s
+  }
 }
 
 template <typename S>
@@ -379,6 +457,7 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
     a->one_minus[k] = p ? (C)p->one_minus[k] : (C)1;
     a->wshift[k] = p ? (C)p->wshift[k] : (C)0;
   }
+  for (int c = 0; c < NCREF; ++c) a->cref[c] = p ? (C)p->cref[c] : (C)0;
   a->half_dt = p ? (C)(0.5 * p->dt) : (C)0;
   a->t = p ? (C)p->t : (C)0;
   return 0;
@@ -457,13 +536,7 @@ int lbm_gen_step(const void* d_f, void* d_fnew, const lbm_layout* layout, const 
                  int periodic_fold, void* stream) {
   LBM_REQUIRE(layout, "layout is NULL");
   hipStream_t s = (hipStream_t)stream;
-  switch (layout->storage) {
-    case LBM_F64: return step_launch<double>(d_f, d_fnew, layout, params, periodic_fold, s);
-    case LBM_F32: return step_launch<float>(d_f, d_fnew, layout, params, periodic_fold, s);
-    case LBM_F16D: return step_launch<_Float16>(d_f, d_fnew, layout, params, periodic_fold, s);
-    case LBM_BF16D: return step_launch<bf16>(d_f, d_fnew, layout, params, periodic_fold, s);
-  }
-  LBM_REQUIRE(false, "unknown storage %%d", layout->storage);
+  LBM_DISPATCH_STORAGE(layout->storage, step_launch, d_f, d_fnew, layout, params, periodic_fold, s);
 }
 
 int lbm_gen_f2m(const void* d_f, const lbm_layout* fl, double* d_m, const lbm_layout* ml,
@@ -471,13 +544,7 @@ int lbm_gen_f2m(const void* d_f, const lbm_layout* fl, double* d_m, const lbm_la
   int rc = same_extents(fl, ml);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  switch (fl->storage) {
-    case LBM_F64: return f2m_launch<double>(d_f, fl, d_m, ml, params, s);
-    case LBM_F32: return f2m_launch<float>(d_f, fl, d_m, ml, params, s);
-    case LBM_F16D: return f2m_launch<_Float16>(d_f, fl, d_m, ml, params, s);
-    case LBM_BF16D: return f2m_launch<bf16>(d_f, fl, d_m, ml, params, s);
-  }
-  LBM_REQUIRE(false, "unknown storage");
+  LBM_DISPATCH_STORAGE(fl->storage, f2m_launch, d_f, fl, d_m, ml, params, s);
 }
 
 int lbm_gen_m2f(const double* d_m, const lbm_layout* ml, void* d_f, const lbm_layout* fl,
@@ -485,13 +552,7 @@ int lbm_gen_m2f(const double* d_m, const lbm_layout* ml, void* d_f, const lbm_la
   int rc = same_extents(fl, ml);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  switch (fl->storage) {
-    case LBM_F64: return m2f_launch<double>(d_m, ml, d_f, fl, params, s);
-    case LBM_F32: return m2f_launch<float>(d_m, ml, d_f, fl, params, s);
-    case LBM_F16D: return m2f_launch<_Float16>(d_m, ml, d_f, fl, params, s);
-    case LBM_BF16D: return m2f_launch<bf16>(d_m, ml, d_f, fl, params, s);
-  }
-  LBM_REQUIRE(false, "unknown storage");
+  LBM_DISPATCH_STORAGE(fl->storage, m2f_launch, d_m, ml, d_f, fl, params, s);
 }
 
 int lbm_gen_equilibrium(double* d_m, const lbm_layout* ml, void* stream) {
@@ -564,7 +625,8 @@ def generate_source(spec: KernelSpec) -> str:
         loads.append("C f%d;" % k)
     loads.append("if (!FOLD) {")
     for k in range(ns):
-        loads.append("  f%d = Codec<S>::load(f + a.src_off[%d], (uint32_t)cell, a.wshift[%d]);" % (k, k, k))
+        loads.append("  f%d = DEV ? Codec<S>::load_dev(f + a.src_off[%d], (uint32_t)cell, a.wshift[%d])"
+                     " : Codec<S>::load(f + a.src_off[%d], (uint32_t)cell, a.wshift[%d]);" % (k, k, k, k, k))
     loads.append("} else {")
     # wrapped coordinates per distinct component value
     comps = [sorted({v[d] for v in vel}) for d in range(3)]
@@ -578,19 +640,27 @@ def generate_source(spec: KernelSpec) -> str:
             loads.append("  %s += (%s < a.halo[%d]) ? a.nin[%d] : 0;" % (nm, nm, d, d))
             loads.append("  %s -= (%s >= a.halo[%d] + a.nin[%d]) ? a.nin[%d] : 0;" % (nm, nm, d, d, d))
     for k, (v0, v1, v2) in enumerate(vel):
-        loads.append("  f%d = Codec<S>::load(f + a.dst_off[%d], (uint32_t)((%s * a.n1 + %s) * a.pitch + a.lead + %s), a.wshift[%d]);"
-                     % (k, k, names[(0, v0)], names[(1, v1)], names[(2, v2)], k))
+        idx = "(uint32_t)((%s * a.n1 + %s) * a.pitch + a.lead + %s)" % (names[(0, v0)], names[(1, v1)], names[(2, v2)])
+        loads.append("  f%d = DEV ? Codec<S>::load_dev(f + a.dst_off[%d], %s, a.wshift[%d])"
+                     " : Codec<S>::load(f + a.dst_off[%d], %s, a.wshift[%d]);" % (k, k, idx, k, k, idx, k))
     loads.append("}")
 
     b1 = _SumBuilder("p")
     f2m_lines = _transform(b1, M, fvars, mvars, ksym, sig_f2m)
     f2m_block = b1.lines + ["C %s = %s;" % (o, e) for o, e in f2m_lines]
     coll = _collision_lines(spec, with_sources=True)
+    coll_dev, dm_names = _deviation_collision_lines(spec)
+    bd = _SumBuilder("rd")
+    m2f_dev_lines = _inverse_transform(bd, iM, dm_names, ["g%d" % k for k in range(ns)], ksym, sig_m2f,
+                                       base=fvars, tag="d")
+    m2f_dev_block = bd.lines + ["const C %s = %s;" % (o, e) for o, e in m2f_dev_lines]
     b2 = _SumBuilder("r")
     m2f_lines = _inverse_transform(b2, iM, mvars, ["g%d" % k for k in range(ns)], ksym, sig_m2f)
     m2f_block = b2.lines + ["const C %s = %s;" % (o, e) for o, e in m2f_lines]
     stores = ["Codec<S>::store(fn + a.dst_off[%d], (uint32_t)cell, g%d, a.wshift[%d]);" % (k, k, k)
               for k in range(ns)]
+    stores_dev = ["Codec<S>::store_dev(fn + a.dst_off[%d], (uint32_t)cell, g%d, a.wshift[%d]);" % (k, k, k)
+                  for k in range(ns)]
 
     # ---------------------------------------------------------- f2m / m2f / eq
     b3 = _SumBuilder("p")
@@ -616,8 +686,9 @@ def generate_source(spec: KernelSpec) -> str:
     return _TEMPLATE % dict(
         hash=spec_hash(spec), ns=ns, nc=spec.nconsm, dim=dim, paired=bool(sig_f2m),
         vel=vel_txt,
-        loads=_indent(loads), f2m=_indent(f2m_block), coll=_indent(coll), m2f=_indent(m2f_block),
-        stores=_indent(stores),
+        loads=_indent(loads), f2m=_indent(f2m_block), coll=_indent(coll, 4), coll_dev=_indent(coll_dev, 4),
+        m2f=_indent(m2f_block, 4), m2f_dev=_indent(m2f_dev_block, 4), stores_dev=_indent(stores_dev, 4),
+        stores=_indent(stores, 4),
         f2m_loads=_indent(f2m_loads), f2m_full=_indent(f2m_full), f2m_stores=_indent(f2m_stores),
         m2f_loads=_indent(m2f_loads), m2f_full=_indent(m2f_full), m2f_stores=_indent(m2f_stores),
         eq_loads=_indent(eq_loads), eq_body=_indent(eq_body), eq_stores=_indent(eq_stores),

@@ -12,11 +12,13 @@ import threading
 import numpy as np
 
 __all__ = ["HipUnavailable", "HipError", "LbmLayout", "LbmVelocities", "LbmParams", "LbmLinkSet",
-           "get", "lib_path", "LBM_F64", "LBM_F32", "LBM_F16D", "LBM_BF16D", "MAX_Q"]
+           "get", "lib_path", "LBM_F64", "LBM_F32", "LBM_F16D", "LBM_BF16D", "LBM_F32D", "MAX_Q",
+           "DEVIATION_FORMATS"]
 
-LBM_F64, LBM_F32, LBM_F16D, LBM_BF16D = 0, 1, 2, 3
+LBM_F64, LBM_F32, LBM_F16D, LBM_BF16D, LBM_F32D = 0, 1, 2, 3, 4
 MAX_Q = 32
-ELEM_SIZE = {LBM_F64: 8, LBM_F32: 4, LBM_F16D: 2, LBM_BF16D: 2}
+ELEM_SIZE = {LBM_F64: 8, LBM_F32: 4, LBM_F16D: 2, LBM_BF16D: 2, LBM_F32D: 4}
+DEVIATION_FORMATS = (LBM_F16D, LBM_BF16D, LBM_F32D)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -46,7 +48,8 @@ class LbmVelocities(ctypes.Structure):
 
 class LbmParams(ctypes.Structure):
     _fields_ = [("omega", ctypes.c_double * MAX_Q), ("one_minus", ctypes.c_double * MAX_Q),
-                ("wshift", ctypes.c_double * MAX_Q), ("dt", ctypes.c_double), ("t", ctypes.c_double)]
+                ("wshift", ctypes.c_double * MAX_Q), ("cref", ctypes.c_double * MAX_Q),
+                ("dt", ctypes.c_double), ("t", ctypes.c_double)]
 
 
 class LbmLinkSet(ctypes.Structure):

@@ -13,8 +13,9 @@ Mirrors pylbm/simulation.py:89-420 for ``generator='hip'``:
 Extensions that keep the reference semantics: ``run(nsteps)`` performs
 nsteps complete steps in one call into the library (no Python round-trip per
 step); ``dtype`` selects the population storage ('float64' FP64, 'float32',
-'float16' / 'bfloat16' = 16-bit deviations f_k - w_k rho0 with FP32
-arithmetic); ``conserved_sums()`` returns FP64 compensated global sums.
+'float32-shifted' / 'float16' / 'bfloat16' = deviations f_k - w_k rho0 in
+FP32 / FP16 / BF16; every non-FP64 format computes the collision in FP32 on
+deviations from the rest state); ``conserved_sums()`` returns FP64 compensated global sums.
 """
 import ctypes
 import math
@@ -126,7 +127,11 @@ class Simulation:
             self.params.one_minus[i] = 1.0 - sv
         self.params.dt = self.dt
         vmax = list(self.domain.stencil.vmax)
-        wshift = self._rest_populations() if self.storage in (rtm.LBM_F16D, rtm.LBM_BF16D) else np.zeros(self.ns)
+        # reference state of the FP32-compute formats: conserved moments
+        # (rho0, 0, ...), its equilibrium populations w_k rho0
+        for c in range(self.spec.nconsm):
+            self.params.cref[c] = self.rho0 if c == 0 else 0.0
+        wshift = self._rest_populations() if self.storage != rtm.LBM_F64 else np.zeros(self.ns)
         for k in range(self.ns):
             self.params.wshift[k] = wshift[k]
         self.wshift = wshift
@@ -382,7 +387,7 @@ class Simulation:
         nb = ctypes.c_int64(0)
         self._check(self.rt.lib.lbm_reduce_blocks(ctypes.byref(f.layout), ctypes.byref(nb)), "reduce")
         part = self.rt.malloc(8 * 2 * self.ns * (nb.value + 1))
-        wptr = self.rt.upload(self.wshift.astype(np.float64)) if self.storage in (rtm.LBM_F16D, rtm.LBM_BF16D) else None
+        wptr = self.rt.upload(self.wshift.astype(np.float64)) if self.storage in rtm.DEVIATION_FORMATS else None
         s = (ctypes.c_double * self.ns)()
         c = (ctypes.c_double * self.ns)()
         try:

@@ -12,17 +12,19 @@ Loo.py arrays' ``.get()`` (pylbm/storage.py:126-157).
 import numpy as np
 import sympy as sp
 
-from .runtime import ELEM_SIZE, LBM_BF16D, LBM_F16D, LBM_F32, LBM_F64, LbmLayout
+from .runtime import DEVIATION_FORMATS, ELEM_SIZE, LBM_BF16D, LBM_F16D, LBM_F32, LBM_F32D, LBM_F64, LbmLayout
 
 __all__ = ["make_layout", "DeviceArray", "STORAGE_BY_DTYPE"]
 
 STORAGE_BY_DTYPE = {
     "float64": LBM_F64, "double": LBM_F64,
     "float32": LBM_F32, "single": LBM_F32,
-    "float16": LBM_F16D, "f16-shifted": LBM_F16D,
-    "bfloat16": LBM_BF16D, "bf16-shifted": LBM_BF16D,
+    "float32-shifted": LBM_F32D, "f32-shifted": LBM_F32D,
+    "float16": LBM_F16D, "f16-shifted": LBM_F16D, "float16-shifted": LBM_F16D,
+    "bfloat16": LBM_BF16D, "bf16-shifted": LBM_BF16D, "bfloat16-shifted": LBM_BF16D,
 }
-_NP_STORE = {LBM_F64: np.float64, LBM_F32: np.float32, LBM_F16D: np.float16, LBM_BF16D: np.uint16}
+_NP_STORE = {LBM_F64: np.float64, LBM_F32: np.float32, LBM_F16D: np.float16, LBM_BF16D: np.uint16,
+             LBM_F32D: np.float32}
 
 
 def make_layout(ns, shape_halo, halo, storage):
@@ -124,7 +126,7 @@ class DeviceArray:
             vals = bf16_to_f32(raw).astype(np.float64)
         else:
             vals = raw.astype(np.float64)
-        if self.storage in (LBM_F16D, LBM_BF16D):
+        if self.storage in DEVIATION_FORMATS:
             count = raw.shape[0]
             vals += self.wshift[first:first + count].reshape((count,) + (1,) * self.dim)
         return vals
@@ -134,7 +136,7 @@ class DeviceArray:
         values = np.asarray(values, dtype=np.float64)
         count = values.shape[0]
         L = self.layout
-        if self.storage in (LBM_F16D, LBM_BF16D):
+        if self.storage in DEVIATION_FORMATS:
             values = values - self.wshift[first:first + count].reshape((count,) + (1,) * self.dim)
         if self.storage == LBM_BF16D:
             stored = f32_to_bf16(values.astype(np.float32))
