@@ -50,6 +50,19 @@ __device__ __forceinline__ bf16 float_to_bf16(float x) {
   return r;
 }
 
+// Population stores; LBM_NT_STORES=1 makes them non-temporal (streamed past the caches).
+#ifndef LBM_NT_STORES
+#define LBM_NT_STORES 0
+#endif
+template <typename T>
+__device__ __forceinline__ void put(T* p, T v) {
+#if LBM_NT_STORES
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // FP32 deviation storage (distinct type so it gets its own codec)
 struct f32dev {
   float x;
@@ -67,10 +80,14 @@ template <>
 struct Codec<double> {
   using C = double;
   static constexpr int kind = LBM_F64;
+  __device__ __forceinline__ static C decode(double r, C) { return r; }
+  __device__ __forceinline__ static C decode_dev(double r, C w) { return r - w; }
+  __device__ __forceinline__ static double encode(C v, C) { return v; }
+  __device__ __forceinline__ static double encode_dev(C d, C w) { return w + d; }
   __device__ __forceinline__ static C load(const double* p, int64_t i, C) { return p[i]; }
-  __device__ __forceinline__ static void store(double* p, int64_t i, C v, C) { p[i] = v; }
+  __device__ __forceinline__ static void store(double* p, int64_t i, C v, C) { put(p + i, v); }
   __device__ __forceinline__ static C load_dev(const double* p, int64_t i, C w) { return p[i] - w; }
-  __device__ __forceinline__ static void store_dev(double* p, int64_t i, C d, C w) { p[i] = w + d; }
+  __device__ __forceinline__ static void store_dev(double* p, int64_t i, C d, C w) { put(p + i, w + d); }
   __device__ __forceinline__ static double widen(double s, double) { return s; }
 };
 
@@ -78,10 +95,14 @@ template <>
 struct Codec<float> {
   using C = float;
   static constexpr int kind = LBM_F32;
+  __device__ __forceinline__ static C decode(float r, C) { return r; }
+  __device__ __forceinline__ static C decode_dev(float r, C w) { return r - w; }
+  __device__ __forceinline__ static float encode(C v, C) { return v; }
+  __device__ __forceinline__ static float encode_dev(C d, C w) { return w + d; }
   __device__ __forceinline__ static C load(const float* p, int64_t i, C) { return p[i]; }
-  __device__ __forceinline__ static void store(float* p, int64_t i, C v, C) { p[i] = v; }
+  __device__ __forceinline__ static void store(float* p, int64_t i, C v, C) { put(p + i, v); }
   __device__ __forceinline__ static C load_dev(const float* p, int64_t i, C w) { return p[i] - w; }
-  __device__ __forceinline__ static void store_dev(float* p, int64_t i, C d, C w) { p[i] = w + d; }
+  __device__ __forceinline__ static void store_dev(float* p, int64_t i, C d, C w) { put(p + i, w + d); }
   __device__ __forceinline__ static double widen(float s, double) { return (double)s; }
 };
 
@@ -89,10 +110,14 @@ template <>
 struct Codec<f32dev> {
   using C = float;
   static constexpr int kind = LBM_F32D;
+  __device__ __forceinline__ static C decode(f32dev r, C w) { return w + r.x; }
+  __device__ __forceinline__ static C decode_dev(f32dev r, C) { return r.x; }
+  __device__ __forceinline__ static f32dev encode(C v, C w) { return f32dev{v - w}; }
+  __device__ __forceinline__ static f32dev encode_dev(C d, C) { return f32dev{d}; }
   __device__ __forceinline__ static C load(const f32dev* p, int64_t i, C w) { return w + p[i].x; }
-  __device__ __forceinline__ static void store(f32dev* p, int64_t i, C v, C w) { p[i].x = v - w; }
+  __device__ __forceinline__ static void store(f32dev* p, int64_t i, C v, C w) { put(&p[i].x, v - w); }
   __device__ __forceinline__ static C load_dev(const f32dev* p, int64_t i, C) { return p[i].x; }
-  __device__ __forceinline__ static void store_dev(f32dev* p, int64_t i, C d, C) { p[i].x = d; }
+  __device__ __forceinline__ static void store_dev(f32dev* p, int64_t i, C d, C) { put(&p[i].x, d); }
   __device__ __forceinline__ static double widen(f32dev s, double w) { return w + (double)s.x; }
 };
 
@@ -100,14 +125,18 @@ template <>
 struct Codec<_Float16> {
   using C = float;
   static constexpr int kind = LBM_F16D;
+  __device__ __forceinline__ static C decode(_Float16 r, C w) { return w + (float)r; }
+  __device__ __forceinline__ static C decode_dev(_Float16 r, C) { return (float)r; }
+  __device__ __forceinline__ static _Float16 encode(C v, C w) { return (_Float16)(v - w); }
+  __device__ __forceinline__ static _Float16 encode_dev(C d, C) { return (_Float16)d; }
   __device__ __forceinline__ static C load(const _Float16* p, int64_t i, C w) {
     return w + (float)p[i];
   }
   __device__ __forceinline__ static void store(_Float16* p, int64_t i, C v, C w) {
-    p[i] = (_Float16)(v - w);
+    put(p + i, (_Float16)(v - w));
   }
   __device__ __forceinline__ static C load_dev(const _Float16* p, int64_t i, C) { return (float)p[i]; }
-  __device__ __forceinline__ static void store_dev(_Float16* p, int64_t i, C d, C) { p[i] = (_Float16)d; }
+  __device__ __forceinline__ static void store_dev(_Float16* p, int64_t i, C d, C) { put(p + i, (_Float16)d); }
   __device__ __forceinline__ static double widen(_Float16 s, double w) { return w + (double)(float)s; }
 };
 
@@ -115,18 +144,54 @@ template <>
 struct Codec<bf16> {
   using C = float;
   static constexpr int kind = LBM_BF16D;
+  __device__ __forceinline__ static C decode(bf16 r, C w) { return w + bf16_to_float(r); }
+  __device__ __forceinline__ static C decode_dev(bf16 r, C) { return bf16_to_float(r); }
+  __device__ __forceinline__ static bf16 encode(C v, C w) { return float_to_bf16(v - w); }
+  __device__ __forceinline__ static bf16 encode_dev(C d, C) { return float_to_bf16(d); }
   __device__ __forceinline__ static C load(const bf16* p, int64_t i, C w) {
     return w + bf16_to_float(p[i]);
   }
   __device__ __forceinline__ static void store(bf16* p, int64_t i, C v, C w) {
-    p[i] = float_to_bf16(v - w);
+    put(&p[i].bits, float_to_bf16(v - w).bits);
   }
   __device__ __forceinline__ static C load_dev(const bf16* p, int64_t i, C) { return bf16_to_float(p[i]); }
-  __device__ __forceinline__ static void store_dev(bf16* p, int64_t i, C d, C) { p[i] = float_to_bf16(d); }
+  __device__ __forceinline__ static void store_dev(bf16* p, int64_t i, C d, C) { put(&p[i].bits, float_to_bf16(d).bits); }
   __device__ __forceinline__ static double widen(bf16 s, double w) {
     return w + (double)bf16_to_float(s);
   }
 };
+
+// ------------------------------------------------- vector load / store
+// VEC consecutive stored populations per thread.  Loads may be aligned to one
+// element only (pulled rows are shifted by the velocity); stores are aligned to
+// the whole vector (interior rows start 128-byte aligned, threads own VEC-aligned
+// runs).
+template <typename S> struct Bits;
+template <> struct Bits<double> { using T = uint64_t; };
+template <> struct Bits<float> { using T = uint32_t; };
+template <> struct Bits<f32dev> { using T = uint32_t; };
+template <> struct Bits<_Float16> { using T = uint16_t; };
+template <> struct Bits<bf16> { using T = uint16_t; };
+
+template <typename S, int V>
+__device__ __forceinline__ void vload(S (&out)[V], const S* p) {
+  __builtin_memcpy(out, p, sizeof(S) * V);
+}
+
+template <typename S, int V>
+__device__ __forceinline__ void vstore(S* p, const S (&in)[V]) {
+  using B = typename Bits<S>::T;
+  if constexpr (V == 1) {
+    B b;
+    __builtin_memcpy(&b, in, sizeof(S));
+    put((B*)p, b);
+  } else {
+    typedef B vec_t __attribute__((ext_vector_type(V)));
+    vec_t v;
+    __builtin_memcpy(&v, in, sizeof(S) * V);
+    put((vec_t*)p, v);
+  }
+}
 
 inline int storage_elem_size(int storage) {
   switch (storage) {

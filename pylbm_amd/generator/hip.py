@@ -361,15 +361,32 @@ struct StepArgs {
   C half_dt, t;
 };
 
-template <typename S, bool FOLD>
-__global__ void __launch_bounds__(256)
+// Launch shape: LBM_BX threads along the contiguous axis x LBM_BY rows per
+// workgroup, each thread updating LBM_MROWS consecutive rows of its column.
+#ifndef LBM_BX
+#define LBM_BX 256
+#endif
+#ifndef LBM_BY
+#define LBM_BY 1
+#endif
+#ifndef LBM_MROWS
+#define LBM_MROWS 1
+#endif
+
+#ifndef LBM_VEC
+#define LBM_VEC 1
+#endif
+
+template <typename S, bool FOLD, int VEC>
+__global__ void __launch_bounds__(LBM_BX * LBM_BY)
 fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Codec<S>::C> a) {
   using C = typename Codec<S>::C;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // full-value arithmetic for FP64 (as pylbm), deviations from the rest state otherwise
+  constexpr bool DEV = Codec<S>::kind != LBM_F64;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) * VEC;
   if (i >= a.nin[2]) return;
-  const int mm = blockIdx.y, oo = blockIdx.z;
-  const int o = oo + a.halo[0], m = mm + a.halo[1], ii = i + a.halo[2];
-  const int cell = (o * a.n1 + m) * a.pitch + a.lead + ii;
+  const int oo = blockIdx.z;
+  const int mbase = (blockIdx.y * blockDim.y + threadIdx.y) * LBM_MROWS;
   const C half_dt = a.half_dt;
   const C tt = a.t;
   (void)half_dt; (void)tt;
@@ -377,9 +394,17 @@ fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename 
   const C* omega = a.omega;
   const C* cref = a.cref;
   (void)cref;
-  // full-value arithmetic for FP64 (as pylbm), deviations from the rest state otherwise
-  constexpr bool DEV = Codec<S>::kind != LBM_F64;
+#pragma unroll
+  for (int r = 0; r < LBM_MROWS; ++r) {
+  const int mm = mbase + r;
+  if (mm >= a.nin[1]) return;
+  const int o = oo + a.halo[0], m = mm + a.halo[1], ii = i + a.halo[2];
+  const int cell = (o * a.n1 + m) * a.pitch + a.lead + ii;
 %(loads)s
+%(gdecl)s
+#pragma unroll
+  for (int j = 0; j < VEC; ++j) {
+%(decode)s
 %(f2m)s
   if constexpr (DEV) {
 %(coll_dev)s
@@ -389,6 +414,9 @@ fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename 
 %(coll)s
 %(m2f)s
 %(stores)s
+  }
+  }
+%(vstores)s
   }
 }
 
@@ -474,14 +502,24 @@ static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_p
   StepArgs<C> a;
   int rc = make_args<C>(L, p, &a);
   if (rc) return rc;
-  const unsigned bx = block_x(a.nin[2]);
-  dim3 grid((unsigned)((a.nin[2] + bx - 1) / bx), (unsigned)a.nin[1], (unsigned)a.nin[0]);
+  const int vec = (a.nin[2] %% LBM_VEC == 0) ? LBM_VEC : 1;
+  const int threads = a.nin[2] / vec;
+  unsigned bx = LBM_BX;
+  while (bx > 64 && (int)(bx / 2) >= threads) bx /= 2;
+  const unsigned rows = LBM_BY * LBM_MROWS;
+  dim3 block(bx, LBM_BY);
+  dim3 grid((unsigned)((threads + bx - 1) / bx), (unsigned)((a.nin[1] + rows - 1) / rows),
+            (unsigned)a.nin[0]);
   if (fold) {
     for (int d = 0; d < 3; ++d)
-      LBM_REQUIRE(a.halo[d] <= 1 || a.nin[d] >= a.halo[d], "periodic fold needs extents >= halo");
-    fused_step<S, true><<<grid, bx, 0, s>>>((const S*)f, (S*)fn, a);
+      LBM_REQUIRE(a.nin[d] >= a.halo[d], "periodic fold needs extents >= halo");
+  }
+  if (vec == LBM_VEC) {
+    if (fold) fused_step<S, true, LBM_VEC><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+    else fused_step<S, false, LBM_VEC><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
   } else {
-    fused_step<S, false><<<grid, bx, 0, s>>>((const S*)f, (S*)fn, a);
+    if (fold) fused_step<S, true, 1><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+    else fused_step<S, false, 1><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
   }
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
@@ -620,30 +658,46 @@ def generate_source(spec: KernelSpec) -> str:
     mvars = ["m%d" % i for i in range(ns)]
 
     # ---------------------------------------------------------- fused step
-    loads = []
-    for k, (v0, v1, v2) in enumerate(vel):
-        loads.append("C f%d;" % k)
+    loads = ["S %s;" % ", ".join("L%d[VEC]" % k for k in range(ns))]
     loads.append("if (!FOLD) {")
     for k in range(ns):
-        loads.append("  f%d = DEV ? Codec<S>::load_dev(f + a.src_off[%d], (uint32_t)cell, a.wshift[%d])"
-                     " : Codec<S>::load(f + a.src_off[%d], (uint32_t)cell, a.wshift[%d]);" % (k, k, k, k, k))
+        loads.append("  vload<S, VEC>(L%d, f + (a.src_off[%d] + cell));" % (k, k))
     loads.append("} else {")
-    # wrapped coordinates per distinct component value
-    comps = [sorted({v[d] for v in vel}) for d in range(3)]
+    # wrapped outer / mid coordinates per distinct component value; the contiguous
+    # axis is wrapped per element, only for the vectors that straddle the row ends
+    comps = [sorted({v[d] for v in vel}) for d in range(2)]
     names = {}
-    for d, axis in enumerate(("o", "m", "ii")):
+    for d, axis in enumerate(("o", "m")):
         for c in comps[d]:
             nm = "w%d_%s" % (d, ("p%d" % c) if c >= 0 else ("n%d" % -c))
             names[(d, c)] = nm
-            # source coordinate axis - c wrapped into the interior [h, h + nin)
             loads.append("  int %s = %s - (%d);" % (nm, axis, c))
             loads.append("  %s += (%s < a.halo[%d]) ? a.nin[%d] : 0;" % (nm, nm, d, d))
             loads.append("  %s -= (%s >= a.halo[%d] + a.nin[%d]) ? a.nin[%d] : 0;" % (nm, nm, d, d, d))
     for k, (v0, v1, v2) in enumerate(vel):
-        idx = "(uint32_t)((%s * a.n1 + %s) * a.pitch + a.lead + %s)" % (names[(0, v0)], names[(1, v1)], names[(2, v2)])
-        loads.append("  f%d = DEV ? Codec<S>::load_dev(f + a.dst_off[%d], %s, a.wshift[%d])"
-                     " : Codec<S>::load(f + a.dst_off[%d], %s, a.wshift[%d]);" % (k, k, idx, k, k, idx, k))
+        loads.append("  {")
+        loads.append("    const int row = a.dst_off[%d] + (%s * a.n1 + %s) * a.pitch + a.lead;"
+                     % (k, names[(0, v0)], names[(1, v1)]))
+        loads.append("    const int s0 = ii - (%d);" % v2)
+        if v2 == 0:
+            loads.append("    vload<S, VEC>(L%d, f + (row + s0));" % k)
+        else:
+            loads.append("    if (s0 >= a.halo[2] && s0 + VEC <= a.halo[2] + a.nin[2]) {")
+            loads.append("      vload<S, VEC>(L%d, f + (row + s0));" % k)
+            loads.append("    } else {")
+            loads.append("      for (int j = 0; j < VEC; ++j) {")
+            loads.append("        int c = s0 + j;")
+            loads.append("        c += (c < a.halo[2]) ? a.nin[2] : 0;")
+            loads.append("        c -= (c >= a.halo[2] + a.nin[2]) ? a.nin[2] : 0;")
+            loads.append("        L%d[j] = f[row + c];" % k)
+            loads.append("      }")
+            loads.append("    }")
+        loads.append("  }")
     loads.append("}")
+    gdecl = ["S %s;" % ", ".join("G%d[VEC]" % k for k in range(ns))]
+    decode = ["C f%d = DEV ? Codec<S>::decode_dev(L%d[j], a.wshift[%d]) : Codec<S>::decode(L%d[j], a.wshift[%d]);"
+              % (k, k, k, k, k) for k in range(ns)]
+    vstores = ["vstore<S, VEC>(fn + (a.dst_off[%d] + cell), G%d);" % (k, k) for k in range(ns)]
 
     b1 = _SumBuilder("p")
     f2m_lines = _transform(b1, M, fvars, mvars, ksym, sig_f2m)
@@ -657,10 +711,8 @@ def generate_source(spec: KernelSpec) -> str:
     b2 = _SumBuilder("r")
     m2f_lines = _inverse_transform(b2, iM, mvars, ["g%d" % k for k in range(ns)], ksym, sig_m2f)
     m2f_block = b2.lines + ["const C %s = %s;" % (o, e) for o, e in m2f_lines]
-    stores = ["Codec<S>::store(fn + a.dst_off[%d], (uint32_t)cell, g%d, a.wshift[%d]);" % (k, k, k)
-              for k in range(ns)]
-    stores_dev = ["Codec<S>::store_dev(fn + a.dst_off[%d], (uint32_t)cell, g%d, a.wshift[%d]);" % (k, k, k)
-                  for k in range(ns)]
+    stores = ["G%d[j] = Codec<S>::encode(g%d, a.wshift[%d]);" % (k, k, k) for k in range(ns)]
+    stores_dev = ["G%d[j] = Codec<S>::encode_dev(g%d, a.wshift[%d]);" % (k, k, k) for k in range(ns)]
 
     # ---------------------------------------------------------- f2m / m2f / eq
     b3 = _SumBuilder("p")
@@ -686,7 +738,8 @@ def generate_source(spec: KernelSpec) -> str:
     return _TEMPLATE % dict(
         hash=spec_hash(spec), ns=ns, nc=spec.nconsm, dim=dim, paired=bool(sig_f2m),
         vel=vel_txt,
-        loads=_indent(loads), f2m=_indent(f2m_block), coll=_indent(coll, 4), coll_dev=_indent(coll_dev, 4),
+        loads=_indent(loads), gdecl=_indent(gdecl), decode=_indent(decode, 4), vstores=_indent(vstores),
+        f2m=_indent(f2m_block, 4), coll=_indent(coll, 4), coll_dev=_indent(coll_dev, 4),
         m2f=_indent(m2f_block, 4), m2f_dev=_indent(m2f_dev_block, 4), stores_dev=_indent(stores_dev, 4),
         stores=_indent(stores, 4),
         f2m_loads=_indent(f2m_loads), f2m_full=_indent(f2m_full), f2m_stores=_indent(f2m_stores),

@@ -49,6 +49,6 @@ def test_printed_transforms_are_exact():
     m2 = rng.random(spec.ns)
     env2 = {"m%d" % i: m2[i] for i in range(spec.ns)}
     start = kern.index("const C r0") if "const C r0" in kern else kern.index("const C E")
-    env2 = _eval_block(kern[start:], kern[start:start + 10], "Codec<S>::store", env2)
+    env2 = _eval_block(kern[start:], kern[start:start + 10], "G0[j] = Codec<S>::encode(", env2)
     g = np.array([env2["g%d" % k] for k in range(spec.ns)])
     assert np.allclose(g, iM @ m2, rtol=1e-13, atol=1e-14)
