@@ -49,14 +49,17 @@ enum lbm_storage {
 #define LBM_MAX_Q 32
 
 /*
- * Device layout of one population array (structure of arrays).
+ * Device layout of one population array.
  * Spatial axes are canonicalised to (outer, mid, inner): 3D (x, y, z),
  * 2D (1, x, y), 1D (1, 1, x) -- the last reference axis is the contiguous one,
  * as in pylbm's SOA storage (pylbm/storage.py:423-472).  Extents include the
  * halo of width halo[a].  Element (k, o, m, i) lives at
- *   k*pop_stride + (o*n[1] + m)*pitch + lead + i
- * with pitch a multiple of 128 bytes and lead chosen so that the first
- * interior element of every row is 128-byte aligned.
+ *   k*pop_stride + (o*n[1] + m)*row_stride + lead + i
+ * with pitch (the padded row length) a multiple of 128 bytes and lead chosen so
+ * that the first interior element of every row is 128-byte aligned.  Two
+ * arrangements: population planes (pop_stride = n0*n1*pitch, row_stride = pitch)
+ * and rows interleaved (pop_stride = pitch, row_stride = ns*pitch: the ns rows
+ * of one (o, m) line are contiguous).
  */
 typedef struct lbm_layout {
   int32_t dim;
@@ -68,7 +71,8 @@ typedef struct lbm_layout {
   int32_t _pad;
   int64_t pitch;       /* elements per inner row (padded) */
   int64_t lead;        /* leading pad elements of each row */
-  int64_t pop_stride;  /* elements between two populations */
+  int64_t pop_stride;  /* elements between two populations of one cell */
+  int64_t row_stride;  /* elements between two consecutive (o, m) rows */
 } lbm_layout;
 
 /* canonical velocity table shared by every kernel of one scheme */

@@ -135,10 +135,17 @@ def _flat(rows, layout):
     dim = rows.shape[0] - 1
     c = [np.zeros(rows.shape[1], dtype=np.int64)] * (3 - dim) + [rows[1 + d].astype(np.int64) for d in range(dim)]
     idx = (rows[0].astype(np.int64) * layout.pop_stride
-           + (c[0] * layout.n[1] + c[1]) * layout.pitch + layout.lead + c[2])
+           + (c[0] * layout.n[1] + c[1]) * layout.row_stride + layout.lead + c[2])
     if idx.size and (idx.min() < 0 or idx.max() >= (1 << 32)):
         raise ValueError("link index out of the 32-bit range")
     return idx
+
+
+def _population(idx, layout):
+    """Population index of flat element indices (either arrangement of the layout)."""
+    if layout.pop_stride > layout.row_stride:
+        return (idx // layout.pop_stride).astype(np.int64)
+    return ((idx // layout.pop_stride) % layout.ns).astype(np.int64)
 
 
 def _resolve_sequential(store, load, sign, rhs):
@@ -183,8 +190,8 @@ class DeviceLinks:
         if self.conflict:
             st, ld, sign, rhs = _resolve_sequential(st, ld, sign, rhs)
             self.n = st.size
-        kstore = (st // layout.pop_stride).astype(np.int64)
-        kload = (ld // layout.pop_stride).astype(np.int64)
+        kstore = _population(st, layout)
+        kload = _population(ld, layout)
         if layout.storage in DEVIATION_FORMATS:
             # deviations: d_s = a d_l + (r + a w_l - w_s)
             rhs = rhs + sign * wshift[kload] - wshift[kstore]
@@ -213,7 +220,8 @@ class DeviceLinks:
         if self.layout.storage in DEVIATION_FORMATS:
             st = _flat(self.method._store_rows(), self.layout)
             ld = _flat(self.method.iload[0].T if self.method._fixed else self.method.iload[0], self.layout)
-            rhs = rhs + self.method.kind.sign * wshift[ld // self.layout.pop_stride] - wshift[st // self.layout.pop_stride]
+            rhs = (rhs + self.method.kind.sign * wshift[_population(ld, self.layout)]
+                   - wshift[_population(st, self.layout)])
         ctype = np.float64 if self.layout.storage == LBM_F64 else np.float32
         if self.d_rhs is None:
             self.d_rhs = self._up(rhs.astype(ctype))

@@ -230,7 +230,7 @@ __host__ __device__ __forceinline__ int64_t interior(const lbm_layout& L, int a)
 // element offset of (o, m, i) inside one population (halo-inclusive indices)
 __host__ __device__ __forceinline__ int64_t cell_offset(const lbm_layout& L, int64_t o, int64_t m,
                                                         int64_t i) {
-  return (o * L.n[1] + m) * L.pitch + L.lead + i;
+  return (o * L.n[1] + m) * L.row_stride + L.lead + i;
 }
 
 inline int check_layout(const lbm_layout* L) {
@@ -244,11 +244,14 @@ inline int check_layout(const lbm_layout* L) {
                 "axis %d: extent %lld halo %d invalid", a, (long long)L->n[a], L->halo[a]);
   }
   LBM_REQUIRE(L->lead >= 0 && L->lead + L->n[2] <= L->pitch, "row lead/pitch too small");
-  LBM_REQUIRE(L->pop_stride >= L->n[0] * L->n[1] * L->pitch, "pop_stride too small");
+  const int64_t rows = L->n[0] * L->n[1];
+  const bool planes = L->row_stride >= L->pitch && L->pop_stride >= rows * L->row_stride;
+  const bool interleaved = L->pop_stride >= L->pitch && L->row_stride >= L->ns * L->pop_stride;
+  LBM_REQUIRE(planes || interleaved, "pop_stride / row_stride overlap");
   // every element index must fit the 32-bit offsets used by the kernels
-  LBM_REQUIRE(L->pop_stride * (int64_t)L->ns < ((int64_t)1 << 31),
-              "array of %lld elements exceeds the 2^31 element index range",
-              (long long)(L->pop_stride * (int64_t)L->ns));
+  const int64_t last = (L->ns - 1) * L->pop_stride + (rows - 1) * L->row_stride + L->pitch;
+  LBM_REQUIRE(last < ((int64_t)1 << 31), "array of %lld elements exceeds the 2^31 element index range",
+              (long long)last);
   return 0;
 }
 

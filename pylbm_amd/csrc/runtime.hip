@@ -47,34 +47,36 @@ __device__ __forceinline__ bool ghost_axis(int64_t c, int64_t n, int h, int64_t*
   return false;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) ghost_fill_kernel(T* __restrict__ f, lbm_layout L,
+// src and dst are the same array: every read is an interior slot and every write a
+// ghost slot, so they never touch the same element and may be declared restrict,
+// which lets all loads of a cell be issued before its stores.
+template <typename T, int QMAX>
+__global__ void __launch_bounds__(256) ghost_fill_kernel(const T* __restrict__ src_f,
+                                                         T* __restrict__ dst_f, lbm_layout L,
                                                          lbm_velocities V, ShellDesc S,
                                                          int all_pops) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t total = S.nA + S.nB + S.nC;
-  if (t >= total) return;
-  const int64_t n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int)(S.nA + S.nB + S.nC)) return;
+  const int n0 = (int)L.n[0], n1 = (int)L.n[1], n2 = (int)L.n[2];
   const int h0 = L.halo[0], h1 = L.halo[1], h2 = L.halo[2];
-  int64_t o, m, i;
-  if (t < S.nA) {  // outer ghost slabs: 2*h0 x n1 x n2
-    int64_t r = t;
+  int o, m, i;
+  if (t < (int)S.nA) {  // outer ghost slabs: 2*h0 x n1 x n2
+    int r = t;
     i = r % n2;
     r /= n2;
     m = r % n1;
     r /= n1;
     o = (r < h0) ? r : (n0 - 2 * h0 + r);
-  } else if (t < S.nA + S.nB) {  // mid ghost slabs: (n0-2h0) x 2*h1 x n2
-    int64_t r = t - S.nA;
+  } else if (t < (int)(S.nA + S.nB)) {  // mid ghost slabs: (n0-2h0) x 2*h1 x n2
+    int r = t - (int)S.nA;
     i = r % n2;
     r /= n2;
-    const int64_t g = r % (2 * h1);
-    r /= (2 * h1);
-    o = h0 + r;
+    const int g = r % (2 * h1);
+    o = h0 + r / (2 * h1);
     m = (g < h1) ? g : (n1 - 2 * h1 + g);
   } else {  // inner ghost slabs: (n0-2h0) x (n1-2h1) x 2*h2
-    int64_t r = t - S.nA - S.nB;
-    const int64_t g = r % (2 * h2);
+    int r = t - (int)(S.nA + S.nB);
+    const int g = r % (2 * h2);
     r /= (2 * h2);
     m = h1 + r % (n1 - 2 * h1);
     o = h0 + r / (n1 - 2 * h1);
@@ -85,18 +87,36 @@ __global__ void __launch_bounds__(256) ghost_fill_kernel(T* __restrict__ f, lbm_
   ghost_axis(o, n0, h0, &wo, &d0);
   ghost_axis(m, n1, h1, &wm, &d1);
   ghost_axis(i, n2, h2, &wi, &d2);
-  const int64_t dst = cell_offset(L, o, m, i);
-  const int64_t src = cell_offset(L, wo, wm, wi);
-  for (int k = 0; k < L.ns; ++k) {
-    bool take = true;
+  const int dst = (int)cell_offset(L, o, m, i);
+  const int src = (int)cell_offset(L, wo, wm, wi);
+  const int ps = (int)L.pop_stride;
+  T val[QMAX];
+  bool take[QMAX];
+#pragma unroll
+  for (int k = 0; k < QMAX; ++k) {
+    take[k] = k < L.ns;
     if (!all_pops) {
       const int c0 = V.v[k][0], c1 = V.v[k][1], c2 = V.v[k][2];
-      take = (d0 == 0 || (d0 > 0 ? c0 >= d0 : c0 <= d0)) &&
-             (d1 == 0 || (d1 > 0 ? c1 >= d1 : c1 <= d1)) &&
-             (d2 == 0 || (d2 > 0 ? c2 >= d2 : c2 <= d2));
+      take[k] = take[k] && (d0 == 0 || (d0 > 0 ? c0 >= d0 : c0 <= d0)) &&
+                (d1 == 0 || (d1 > 0 ? c1 >= d1 : c1 <= d1)) &&
+                (d2 == 0 || (d2 > 0 ? c2 >= d2 : c2 <= d2));
     }
-    if (take) f[k * L.pop_stride + dst] = f[k * L.pop_stride + src];
+    if (take[k]) val[k] = src_f[k * ps + src];
   }
+#pragma unroll
+  for (int k = 0; k < QMAX; ++k)
+    if (take[k]) dst_f[k * ps + dst] = val[k];
+}
+
+template <typename T>
+static void ghost_launch(void* f, const lbm_layout& L, const lbm_velocities& V, const ShellDesc& S,
+                         int all_pops, unsigned blocks, hipStream_t s) {
+  const T* src = (const T*)f;
+  T* dst = (T*)f;
+  if (L.ns <= 4) ghost_fill_kernel<T, 4><<<blocks, 256, 0, s>>>(src, dst, L, V, S, all_pops);
+  else if (L.ns <= 9) ghost_fill_kernel<T, 9><<<blocks, 256, 0, s>>>(src, dst, L, V, S, all_pops);
+  else if (L.ns <= 19) ghost_fill_kernel<T, 19><<<blocks, 256, 0, s>>>(src, dst, L, V, S, all_pops);
+  else ghost_fill_kernel<T, LBM_MAX_Q><<<blocks, 256, 0, s>>>(src, dst, L, V, S, all_pops);
 }
 
 // ------------------------------------------------------------ link lists
@@ -395,9 +415,9 @@ static int ghost_fill_impl(void* d_f, const lbm_layout* L, const lbm_velocities*
   const unsigned blocks = (unsigned)((total + 255) / 256);
   hipStream_t s = (hipStream_t)stream;
   switch (L->elem_size) {
-    case 8: ghost_fill_kernel<uint64_t><<<blocks, 256, 0, s>>>((uint64_t*)d_f, *L, *V, S, all_pops); break;
-    case 4: ghost_fill_kernel<uint32_t><<<blocks, 256, 0, s>>>((uint32_t*)d_f, *L, *V, S, all_pops); break;
-    case 2: ghost_fill_kernel<uint16_t><<<blocks, 256, 0, s>>>((uint16_t*)d_f, *L, *V, S, all_pops); break;
+    case 8: ghost_launch<uint64_t>(d_f, *L, *V, S, all_pops, blocks, s); break;
+    case 4: ghost_launch<uint32_t>(d_f, *L, *V, S, all_pops, blocks, s); break;
+    case 2: ghost_launch<uint16_t>(d_f, *L, *V, S, all_pops, blocks, s); break;
     default: LBM_REQUIRE(false, "bad element size %d", L->elem_size);
   }
   LBM_CHECK_HIP(hipGetLastError());

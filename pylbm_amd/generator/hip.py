@@ -338,6 +338,10 @@ def _equilibrium_lines(spec: KernelSpec):
 # ------------------------------------------------------------------ template
 _TEMPLATE = r"""// Generated by pylbm_amd.generator.hip for scheme %(hash)s -- do not edit.
 // ns=%(ns)d nconsm=%(nc)d dim=%(dim)d pair-factored=%(paired)s
+// Population stores bypass the caches (each value is read once, in the next step).
+#ifndef LBM_NT_STORES
+#define LBM_NT_STORES 1
+#endif
 #include "lbm_device.h"
 
 using namespace lbm;
@@ -355,7 +359,7 @@ struct StepArgs {
   int32_t dst_off[NS];   // k * pop_stride
   int32_t nin[3];        // interior extents
   int32_t halo[3];
-  int32_t n1, pitch, lead, _pad;
+  int32_t n1, rs, lead, _pad;      // rs: elements between consecutive rows
   C omega[NS], one_minus[NS], wshift[NS];
   C cref[NCREF];
   C half_dt, t;
@@ -373,9 +377,17 @@ struct StepArgs {
 #define LBM_MROWS 1
 #endif
 
+// Cells per thread along the contiguous axis (vector loads / stores): 0 = by
+// element size, measured on MI355X: 4 cells (16 B per lane) for the 4-byte
+// formats, 1 for FP64 (no gain) and for the 16-bit formats (their pulled rows
+// are only 2-byte aligned and the vector loads split).
 #ifndef LBM_VEC
-#define LBM_VEC 1
+#define LBM_VEC 0
 #endif
+template <typename S>
+constexpr int vec_cells() {
+  return LBM_VEC > 0 ? LBM_VEC : (sizeof(S) == 4 ? 4 : 1);
+}
 
 template <typename S, bool FOLD, int VEC>
 __global__ void __launch_bounds__(LBM_BX * LBM_BY)
@@ -399,7 +411,7 @@ fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename 
   const int mm = mbase + r;
   if (mm >= a.nin[1]) return;
   const int o = oo + a.halo[0], m = mm + a.halo[1], ii = i + a.halo[2];
-  const int cell = (o * a.n1 + m) * a.pitch + a.lead + ii;
+  const int cell = (o * a.n1 + m) * a.rs + a.lead + ii;
 %(loads)s
 %(gdecl)s
 #pragma unroll
@@ -470,14 +482,14 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   static const int hv[NS][3] = {%(vel)s};
   for (int k = 0; k < NS; ++k) {
     a->dst_off[k] = (int32_t)(k * L->pop_stride);
-    a->src_off[k] = (int32_t)(k * L->pop_stride - ((int64_t)hv[k][0] * L->n[1] + hv[k][1]) * L->pitch - hv[k][2]);
+    a->src_off[k] = (int32_t)(k * L->pop_stride - ((int64_t)hv[k][0] * L->n[1] + hv[k][1]) * L->row_stride - hv[k][2]);
   }
   for (int d = 0; d < 3; ++d) {
     a->nin[d] = (int32_t)(L->n[d] - 2 * L->halo[d]);
     a->halo[d] = L->halo[d];
   }
   a->n1 = (int32_t)L->n[1];
-  a->pitch = (int32_t)L->pitch;
+  a->rs = (int32_t)L->row_stride;
   a->lead = (int32_t)L->lead;
   a->_pad = 0;
   for (int k = 0; k < NS; ++k) {
@@ -502,7 +514,8 @@ static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_p
   StepArgs<C> a;
   int rc = make_args<C>(L, p, &a);
   if (rc) return rc;
-  const int vec = (a.nin[2] %% LBM_VEC == 0) ? LBM_VEC : 1;
+  constexpr int V = vec_cells<S>();
+  const int vec = (a.nin[2] %% V == 0) ? V : 1;
   const int threads = a.nin[2] / vec;
   unsigned bx = LBM_BX;
   while (bx > 64 && (int)(bx / 2) >= threads) bx /= 2;
@@ -514,9 +527,9 @@ static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_p
     for (int d = 0; d < 3; ++d)
       LBM_REQUIRE(a.nin[d] >= a.halo[d], "periodic fold needs extents >= halo");
   }
-  if (vec == LBM_VEC) {
-    if (fold) fused_step<S, true, LBM_VEC><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
-    else fused_step<S, false, LBM_VEC><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+  if (vec == V) {
+    if (fold) fused_step<S, true, V><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+    else fused_step<S, false, V><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
   } else {
     if (fold) fused_step<S, true, 1><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
     else fused_step<S, false, 1><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
@@ -676,7 +689,7 @@ def generate_source(spec: KernelSpec) -> str:
             loads.append("  %s -= (%s >= a.halo[%d] + a.nin[%d]) ? a.nin[%d] : 0;" % (nm, nm, d, d, d))
     for k, (v0, v1, v2) in enumerate(vel):
         loads.append("  {")
-        loads.append("    const int row = a.dst_off[%d] + (%s * a.n1 + %s) * a.pitch + a.lead;"
+        loads.append("    const int row = a.dst_off[%d] + (%s * a.n1 + %s) * a.rs + a.lead;"
                      % (k, names[(0, v0)], names[(1, v1)]))
         loads.append("    const int s0 = ii - (%d);" % v2)
         if v2 == 0:

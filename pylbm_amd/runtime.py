@@ -39,7 +39,7 @@ class LbmLayout(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("ns", ctypes.c_int32), ("storage", ctypes.c_int32),
                 ("elem_size", ctypes.c_int32), ("n", ctypes.c_int64 * 3), ("halo", ctypes.c_int32 * 3),
                 ("_pad", ctypes.c_int32), ("pitch", ctypes.c_int64), ("lead", ctypes.c_int64),
-                ("pop_stride", ctypes.c_int64)]
+                ("pop_stride", ctypes.c_int64), ("row_stride", ctypes.c_int64)]
 
 
 class LbmVelocities(ctypes.Structure):

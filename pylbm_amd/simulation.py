@@ -71,9 +71,9 @@ class _Container:
 
     gpu_support = True
 
-    def __init__(self, rt, ns, shape_halo, vmax, storage, consm, wshift):
-        self.F = DeviceArray(rt, ns, shape_halo, vmax, storage, wshift)
-        self.Fnew = DeviceArray(rt, ns, shape_halo, vmax, storage, wshift)
+    def __init__(self, rt, ns, shape_halo, vmax, storage, consm, wshift, interleaved=True):
+        self.F = DeviceArray(rt, ns, shape_halo, vmax, storage, wshift, interleaved)
+        self.Fnew = DeviceArray(rt, ns, shape_halo, vmax, storage, wshift, interleaved)
         self._m = None
         self._rt, self._shape, self._vmax, self._consm = rt, shape_halo, vmax, consm
         for a in (self.F, self.Fnew):
@@ -93,7 +93,7 @@ class Simulation:
     """Lattice Boltzmann simulation on one MI355X (pylbm.Simulation, generator 'hip')."""
 
     def __init__(self, dico, sorder=None, dtype="float64", check_inverse=False, initialize=True,
-                 rho0=1.0, device=0):
+                 rho0=1.0, device=0, layout="interleaved"):
         gen = str(dico.get("generator", "hip")).upper()
         if gen != "HIP":
             raise ValueError("pylbm_amd provides the 'hip' generator only (got %r)" % dico.get("generator"))
@@ -135,8 +135,10 @@ class Simulation:
         for k in range(self.ns):
             self.params.wshift[k] = wshift[k]
         self.wshift = wshift
+        if layout not in ("interleaved", "planes"):
+            raise ValueError("layout must be 'interleaved' or 'planes'")
         self.container = _Container(self.rt, self.ns, self.domain.shape_halo, vmax, self.storage,
-                                    self.scheme.consm, wshift)
+                                    self.scheme.consm, wshift, interleaved=(layout == "interleaved"))
         self.bc = Boundary(self.domain, dico)
         for method in self.bc.methods:
             method.set_iload()

@@ -27,8 +27,12 @@ _NP_STORE = {LBM_F64: np.float64, LBM_F32: np.float32, LBM_F16D: np.float16, LBM
              LBM_F32D: np.float32}
 
 
-def make_layout(ns, shape_halo, halo, storage):
-    """lbm_layout of an array with ``ns`` fields over ``shape_halo`` (reference axes)."""
+def make_layout(ns, shape_halo, halo, storage, interleaved=False):
+    """lbm_layout of an array with ``ns`` fields over ``shape_halo`` (reference axes).
+
+    interleaved=False: population planes (each field contiguous);
+    interleaved=True: the ns rows of every (o, m) line contiguous, which keeps
+    the streams of one workgroup within a few DRAM pages."""
     dim = len(shape_halo)
     n = [1] * (3 - dim) + [int(s) for s in shape_halo]
     h = [0] * (3 - dim) + [int(v) for v in halo]
@@ -42,8 +46,22 @@ def make_layout(ns, shape_halo, halo, storage):
         L.n[a] = n[a]
         L.halo[a] = h[a]
     L.pitch, L.lead = pitch, lead
-    L.pop_stride = n[0] * n[1] * pitch
+    if interleaved:
+        L.pop_stride = pitch
+        L.row_stride = ns * pitch
+    else:
+        L.pop_stride = n[0] * n[1] * pitch
+        L.row_stride = pitch
     return L
+
+
+def layout_elements(L):
+    """Number of elements the array of layout L spans."""
+    return L.ns * L.n[0] * L.n[1] * L.pitch
+
+
+def is_interleaved(L):
+    return L.pop_stride < L.row_stride
 
 
 def bf16_to_f32(bits):
@@ -59,14 +77,15 @@ def f32_to_bf16(x):
 class DeviceArray:
     """``ns`` fields on the halo-inclusive grid, stored on the GPU."""
 
-    def __init__(self, rt, ns, shape_halo, halo, storage=LBM_F64, wshift=None):
+    def __init__(self, rt, ns, shape_halo, halo, storage=LBM_F64, wshift=None, interleaved=False):
         self.rt = rt
         self.ns = ns
         self.shape_halo = tuple(int(s) for s in shape_halo)
         self.vmax = tuple(int(h) for h in halo)
         self.storage = storage
-        self.layout = make_layout(ns, shape_halo, halo, storage)
-        self.nbytes = self.layout.pop_stride * ns * self.layout.elem_size
+        self.layout = make_layout(ns, shape_halo, halo, storage, interleaved)
+        self.interleaved = bool(interleaved)
+        self.nbytes = layout_elements(self.layout) * self.layout.elem_size
         self.ptr = rt.malloc(self.nbytes)
         self.consm = {}
         self.wshift = np.zeros(ns) if wshift is None else np.asarray(wshift, dtype=np.float64)
@@ -110,10 +129,28 @@ class DeviceArray:
         return a.reshape((raw.shape[0],) + self.shape_halo)
 
     # ------------------------------------------------------------ host I/O
+    def _raw_all(self):
+        """Every stored element as (ns, n0, n1, pitch) (storage dtype)."""
+        L = self.layout
+        if not self.interleaved:
+            raw = np.empty(self._padded_shape(self.ns), dtype=_NP_STORE[self.storage])
+            return self.rt.d2h(raw, self.ptr)
+        raw = np.empty((L.n[0], L.n[1], self.ns, L.pitch), dtype=_NP_STORE[self.storage])
+        self.rt.d2h(raw, self.ptr)
+        return np.ascontiguousarray(np.moveaxis(raw, 2, 0))
+
+    def _write_all(self, raw):
+        if not self.interleaved:
+            self.rt.h2d(self.ptr, raw)
+        else:
+            self.rt.h2d(self.ptr, np.ascontiguousarray(np.moveaxis(raw, 0, 2)))
+
     def fields_raw(self, first=0, count=None):
         """Stored values of fields [first, first + count) as a host array (storage dtype)."""
         count = self.ns - first if count is None else count
         L = self.layout
+        if self.interleaved:
+            return self._unpad(self._raw_all()[first:first + count])
         raw = np.empty(self._padded_shape(count), dtype=_NP_STORE[self.storage])
         off = first * L.pop_stride * L.elem_size
         self.rt.d2h(raw, self.ptr + off)
@@ -142,6 +179,11 @@ class DeviceArray:
             stored = f32_to_bf16(values.astype(np.float32))
         else:
             stored = values.astype(_NP_STORE[self.storage])
+        if self.interleaved:
+            raw = self._raw_all() if count < self.ns else np.zeros(self._padded_shape(self.ns), stored.dtype)
+            raw[first:first + count, ..., L.lead:L.lead + L.n[2]] = stored.reshape((count, L.n[0], L.n[1], L.n[2]))
+            self._write_all(raw)
+            return
         raw = np.zeros(self._padded_shape(count), dtype=stored.dtype)
         raw[..., L.lead:L.lead + L.n[2]] = stored.reshape((count, L.n[0], L.n[1], L.n[2]))
         self.rt.h2d(self.ptr + first * L.pop_stride * L.elem_size, raw)

@@ -101,3 +101,24 @@ def test_resolve_sequential_matches_loop():
         par[s2] = gathered
         assert np.allclose(par, seq, rtol=0, atol=1e-12)
         assert np.unique(s2).size == s2.size
+
+
+def test_flat_indices_both_arrangements():
+    """Flat element indices of link rows and their population, for planes and
+    interleaved rows, against a direct numpy index computation."""
+    from pylbm_amd.boundary import _flat, _population
+    from pylbm_amd.runtime import LBM_F32
+    from pylbm_amd.storage import make_layout, layout_elements
+    rng = np.random.default_rng(3)
+    shape = (7, 6, 9)
+    for inter in (False, True):
+        L = make_layout(19, shape, (1, 1, 1), LBM_F32, interleaved=inter)
+        rows = np.stack([rng.integers(0, 19, 200)] + [rng.integers(0, s, 200) for s in shape])
+        idx = _flat(rows, L)
+        if inter:
+            want = ((rows[1] * L.n[1] + rows[2]) * 19 + rows[0]) * L.pitch + L.lead + rows[3]
+        else:
+            want = ((rows[0] * L.n[0] + rows[1]) * L.n[1] + rows[2]) * L.pitch + L.lead + rows[3]
+        assert np.array_equal(idx, want)
+        assert np.array_equal(_population(idx, L), rows[0])
+        assert idx.max() < layout_elements(L)
