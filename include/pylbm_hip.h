@@ -75,10 +75,12 @@ typedef struct lbm_layout {
   int64_t row_stride;  /* elements between two consecutive (o, m) rows */
 } lbm_layout;
 
-/* canonical velocity table shared by every kernel of one scheme */
+/* canonical velocity table shared by every kernel of one scheme, and the
+ * periodic faces: bit 2a (2a+1) set when the low (high) face of canonical axis a
+ * is periodic (pylbm label -1) */
 typedef struct lbm_velocities {
   int32_t q;
-  int32_t _pad;
+  int32_t periodic_faces;
   int8_t v[LBM_MAX_Q][3];  /* canonical (outer, mid, inner) components */
 } lbm_velocities;
 
@@ -107,16 +109,19 @@ int lbm_event_elapsed_ms(void* start, void* stop, float* ms);
 int lbm_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
 
 /*
- * Periodic ghost fill of every halo slot a pull step or a link list can read:
- * each ghost cell receives, for every population entering the interior through
- * all of the cell's ghost axes, the value at the fully wrapped interior
- * position.  Equal, on those slots, to pylbm's axis-by-axis wrap of the whole
- * halo (pylbm/storage.py:306-367; Loo.py kernels update_x/y/z 369-420).
+ * Periodic ghost fill of every halo slot a pull step can read without a link
+ * list writing it first: a ghost cell whose first ghost axis (in axis order x,
+ * y, z) is a periodic face receives, for every population entering the interior
+ * through all of its ghost axes, the value at the fully wrapped interior
+ * position.  On those slots this equals pylbm's axis-by-axis wrap of the whole
+ * halo (pylbm/storage.py:306-367; Loo.py kernels update_x/y/z 369-420); every
+ * other slot a pull reads belongs to a link of a non-periodic label (pylbm flags
+ * a corner link with its first axis), which its boundary method overwrites.
  */
 int lbm_ghost_fill(void* d_f, const lbm_layout* layout, const lbm_velocities* vel,
                    void* stream);
-/* same, every population of every ghost cell (used when Neumann-type link
- * lists read ghost slots) */
+/* the full axis-by-axis wrap: every population of every ghost cell (used when
+ * Neumann-type link lists read ghost slots) */
 int lbm_ghost_fill_all(void* d_f, const lbm_layout* layout, const lbm_velocities* vel,
                        void* stream);
 

@@ -175,17 +175,25 @@ def _resolve_sequential(store, load, sign, rhs):
 class DeviceLinks:
     """Device arrays of one method for a given population layout."""
 
-    def __init__(self, rt, method, layout, wshift):
+    def __init__(self, rt, method, layout, wshift, flat=None):
+        """flat=(store, load): a plain copy list f[store] = f[load] given as flat
+        element indices (method is then only a name)."""
         self.rt = rt
         self.method = method
         self.layout = layout
-        rows_s = method._store_rows()
-        rows_l = method.iload[0].T if method._fixed else method.iload[0]
-        st = _flat(rows_s, layout)
-        ld = _flat(rows_l, layout)
-        self.n = st.size
-        sign = np.full(self.n, method.kind.sign)
-        rhs = method.rhs.copy() if method.kind.rhs_sign is not None else np.zeros(self.n)
+        if flat is None:
+            rows_s = method._store_rows()
+            rows_l = method.iload[0].T if method._fixed else method.iload[0]
+            st = _flat(rows_s, layout)
+            ld = _flat(rows_l, layout)
+            self.n = st.size
+            sign = np.full(self.n, method.kind.sign)
+            rhs = method.rhs.copy() if method.kind.rhs_sign is not None else np.zeros(self.n)
+        else:
+            st, ld = (np.asarray(a, dtype=np.int64) for a in flat)
+            self.n = st.size
+            sign = np.ones(self.n)
+            rhs = np.zeros(self.n)
         self.conflict = (np.intersect1d(st, ld).size > 0) or (np.unique(st).size != st.size)
         if self.conflict:
             st, ld, sign, rhs = _resolve_sequential(st, ld, sign, rhs)
@@ -245,7 +253,8 @@ class DeviceLinks:
                                         ctypes.c_void_p(self.d_store), ctypes.c_void_p(self.d_load),
                                         ctypes.c_void_p(self.d_rhs), ctypes.c_void_p(self.d_sign),
                                         self.sign, self.n, 1 if self.conflict else 0,
-                                        ctypes.c_void_p(self.d_work), rt.stream), self.method.name)
+                                        ctypes.c_void_p(self.d_work), rt.stream),
+                 getattr(self.method, "name", str(self.method)))
 
     def __del__(self):
         try:
@@ -253,3 +262,55 @@ class DeviceLinks:
                 self.rt.free(p)
         except Exception:
             pass
+
+
+# ------------------------------------------------------ ghost slots outside the fill
+def unwritten_ghost_reads(domain, stencil, methods, layout):
+    """Ghost slots a pull reads that neither the periodic-face fill nor a link list writes.
+
+    pylbm wraps every halo slot before the link lists run (pylbm/storage.py:306-367).
+    The device fill only wraps the slabs of periodic faces (a slot belongs to the
+    slab of its first ghost axis, pylbm's rule for corner links), because the
+    pulled slots of the other slabs are link stores -- except where pylbm dropped
+    the link: cells inside a solid element lose their flags (pylbm/domain.py:
+    578-589, 622-635), so a solid cell next to a wall pulls the wrapped value.
+    Returns flat (store, load) indices of those slots and their wrapped sources."""
+    dim = domain.dim
+    shape = list(domain.shape_halo)
+    h = [int(x) for x in stencil.vmax]
+    v = stencil.get_all_velocities()
+    labels = domain.box_label
+    written = np.concatenate([_flat(m._store_rows(), layout) for m in methods]) if methods else np.zeros(0, np.int64)
+    stores, loads = [], []
+    for a in range(dim):
+        for side in (0, 1):
+            if labels[2 * a + side] == -1 or h[a] == 0:
+                continue
+            ranges = []
+            for b in range(dim):
+                if b < a:
+                    ranges.append(np.arange(h[b], shape[b] - h[b]))
+                elif b == a:
+                    ranges.append(np.arange(0, h[a]) if side == 0 else np.arange(shape[a] - h[a], shape[a]))
+                else:
+                    ranges.append(np.arange(shape[b]))
+            cells = np.stack([g.ravel() for g in np.meshgrid(*ranges, indexing="ij")])
+            for k in range(v.shape[0]):
+                x = cells + v[k][:, None]
+                inside = np.all([(x[b] >= h[b]) & (x[b] < shape[b] - h[b]) for b in range(dim)], axis=0)
+                if not inside.any():
+                    continue
+                s_ = cells[:, inside]
+                wrapped = s_.copy()
+                for b in range(dim):
+                    nin = shape[b] - 2 * h[b]
+                    wrapped[b] = np.where(s_[b] < h[b], s_[b] + nin, np.where(s_[b] >= shape[b] - h[b], s_[b] - nin, s_[b]))
+                kk = np.full((1, s_.shape[1]), k)
+                st = _flat(np.concatenate([kk, s_]), layout)
+                keep = ~np.isin(st, written)
+                if keep.any():
+                    stores.append(st[keep])
+                    loads.append(_flat(np.concatenate([kk, wrapped]), layout)[keep])
+    if not stores:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    return np.concatenate(stores), np.concatenate(loads)

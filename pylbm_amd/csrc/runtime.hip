@@ -27,7 +27,17 @@ void set_error(const char* fmt, ...) {
 // population instead (the full axis-by-axis wrap).
 struct ShellDesc {
   int64_t nA, nB, nC;
+  int sidesA, sidesB, sidesC;  // bit 0: low face slab, bit 1: high face slab
 };
+
+// coordinate of ghost layer g (0 <= g < h * number of sides) along an axis
+__device__ __forceinline__ int slab_coord(int g, int n, int h, int sides) {
+  if (sides == 3) return g < h ? g : n - 2 * h + g;
+  if (sides == 1) return g;
+  return n - h + g;
+}
+
+__host__ __device__ __forceinline__ int nsides(int sides) { return (sides & 1) + ((sides >> 1) & 1); }
 
 __device__ __forceinline__ bool ghost_axis(int64_t c, int64_t n, int h, int64_t* wrapped,
                                            int* depth) {
@@ -60,27 +70,27 @@ __global__ void __launch_bounds__(256) ghost_fill_kernel(const T* __restrict__ s
   const int n0 = (int)L.n[0], n1 = (int)L.n[1], n2 = (int)L.n[2];
   const int h0 = L.halo[0], h1 = L.halo[1], h2 = L.halo[2];
   int o, m, i;
-  if (t < (int)S.nA) {  // outer ghost slabs: 2*h0 x n1 x n2
+  if (t < (int)S.nA) {  // outer ghost slabs: layers x n1 x n2
     int r = t;
     i = r % n2;
     r /= n2;
     m = r % n1;
     r /= n1;
-    o = (r < h0) ? r : (n0 - 2 * h0 + r);
-  } else if (t < (int)(S.nA + S.nB)) {  // mid ghost slabs: (n0-2h0) x 2*h1 x n2
+    o = slab_coord(r, n0, h0, S.sidesA);
+  } else if (t < (int)(S.nA + S.nB)) {  // mid ghost slabs: (n0-2h0) x layers x n2
+    const int nl = h1 * nsides(S.sidesB);
     int r = t - (int)S.nA;
     i = r % n2;
     r /= n2;
-    const int g = r % (2 * h1);
-    o = h0 + r / (2 * h1);
-    m = (g < h1) ? g : (n1 - 2 * h1 + g);
-  } else {  // inner ghost slabs: (n0-2h0) x (n1-2h1) x 2*h2
+    m = slab_coord(r % nl, n1, h1, S.sidesB);
+    o = h0 + r / nl;
+  } else {  // inner ghost slabs: (n0-2h0) x (n1-2h1) x layers
+    const int nl = h2 * nsides(S.sidesC);
     int r = t - (int)(S.nA + S.nB);
-    const int g = r % (2 * h2);
-    r /= (2 * h2);
+    i = slab_coord(r % nl, n2, h2, S.sidesC);
+    r /= nl;
     m = h1 + r % (n1 - 2 * h1);
     o = h0 + r / (n1 - 2 * h1);
-    i = (g < h2) ? g : (n2 - 2 * h2 + g);
   }
   int64_t wo, wm, wi;
   int d0, d1, d2;
@@ -406,10 +416,16 @@ static int ghost_fill_impl(void* d_f, const lbm_layout* L, const lbm_velocities*
   LBM_REQUIRE(V && V->q == L->ns, "velocity table does not match the layout");
   const int64_t n0 = L->n[0], n1 = L->n[1], n2 = L->n[2];
   const int h0 = L->halo[0], h1 = L->halo[1], h2 = L->halo[2];
+  // directional mode: only the slabs of periodic faces (their first ghost axis
+  // is the slab's axis); full mode: every slab
+  const int faces = all_pops ? 0x3F : V->periodic_faces;
   ShellDesc S;
-  S.nA = 2 * h0 * n1 * n2;
-  S.nB = (n0 - 2 * h0) * 2 * h1 * n2;
-  S.nC = (n0 - 2 * h0) * (n1 - 2 * h1) * 2 * h2;
+  S.sidesA = faces & 3;
+  S.sidesB = (faces >> 2) & 3;
+  S.sidesC = (faces >> 4) & 3;
+  S.nA = (int64_t)h0 * nsides(S.sidesA) * n1 * n2;
+  S.nB = (n0 - 2 * h0) * (int64_t)h1 * nsides(S.sidesB) * n2;
+  S.nC = (n0 - 2 * h0) * (n1 - 2 * h1) * (int64_t)h2 * nsides(S.sidesC);
   const int64_t total = S.nA + S.nB + S.nC;
   if (total == 0) return 0;
   const unsigned blocks = (unsigned)((total + 255) / 256);

@@ -43,7 +43,7 @@ class LbmLayout(ctypes.Structure):
 
 
 class LbmVelocities(ctypes.Structure):
-    _fields_ = [("q", ctypes.c_int32), ("_pad", ctypes.c_int32), ("v", (ctypes.c_int8 * 3) * MAX_Q)]
+    _fields_ = [("q", ctypes.c_int32), ("periodic_faces", ctypes.c_int32), ("v", (ctypes.c_int8 * 3) * MAX_Q)]
 
 
 class LbmParams(ctypes.Structure):

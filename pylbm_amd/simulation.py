@@ -26,7 +26,7 @@ import numpy as np
 import sympy as sp
 
 from . import runtime as rtm
-from .boundary import Boundary, DeviceLinks
+from .boundary import Boundary, DeviceLinks, unwritten_ghost_reads
 from .domain import Domain
 from .generator.build import load_module
 from .monitoring import Monitor
@@ -39,9 +39,13 @@ __all__ = ["Simulation"]
 LBM_F64 = rtm.LBM_F64
 
 
-def _canon_velocities(spec):
+def _canon_velocities(spec, box_label=None):
     vel = rtm.LbmVelocities()
     vel.q = spec.ns
+    if box_label is not None:
+        # reference faces (x-, x+, y-, y+, z-, z+) -> canonical axes (outer, mid, inner)
+        shift = 2 * (3 - spec.dim)
+        vel.periodic_faces = sum(1 << (shift + f) for f, lab in enumerate(box_label) if lab == -1)
     for k, v in enumerate(spec.velocities):
         c = [0] * (3 - spec.dim) + list(v)
         for a in range(3):
@@ -119,7 +123,7 @@ class Simulation:
         self.spec = self.scheme.kernel_spec()
         self.module = load_module(self.spec)
         self.ns = self.spec.ns
-        self.vel = _canon_velocities(self.spec)
+        self.vel = _canon_velocities(self.spec, self.domain.box_label)
         self.params = rtm.LbmParams()
         for i, s in enumerate(self.spec.s):
             sv = float(s)
@@ -207,6 +211,13 @@ class Simulation:
             method.set_rhs()
         self._dev_links = [DeviceLinks(self.rt, m, self.container.F.layout, self.wshift)
                            for m in self.bc.methods]
+        if self._ghost_mode == 1:
+            st, ld = unwritten_ghost_reads(self.domain, self.scheme.stencil, self.bc.methods,
+                                           self.container.F.layout)
+            if st.size:
+                # before the link lists: pylbm's wrap happens before its boundary methods
+                self._dev_links.insert(0, DeviceLinks(self.rt, "ghost_copy", self.container.F.layout,
+                                                      self.wshift, flat=(st, ld)))
         self._link_sets = (rtm.LbmLinkSet * max(1, len(self._dev_links)))(
             *[d.link_set() for d in self._dev_links])
         self._need_init = False
@@ -290,7 +301,11 @@ class Simulation:
         fn = self.rt.lib.lbm_ghost_fill if self._ghost_mode == 1 else self.rt.lib.lbm_ghost_fill_all
         self._check(fn(ctypes.c_void_p(f.ptr), ctypes.byref(f.layout), ctypes.byref(self.vel),
                        self.rt.stream), "ghost fill")
-        for method, dev in zip(self.bc.methods, self._dev_links):
+        devs = self._dev_links
+        if len(devs) > len(self.bc.methods):
+            devs[0].apply(f.ptr)
+            devs = devs[1:]
+        for method, dev in zip(self.bc.methods, devs):
             self._update_feq(method, dev)
             dev.apply(f.ptr)
 

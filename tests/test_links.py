@@ -122,3 +122,75 @@ def test_flat_indices_both_arrangements():
         assert np.array_equal(idx, want)
         assert np.array_equal(_population(idx, L), rows[0])
         assert idx.max() < layout_elements(L)
+
+
+class _HostLayout:
+    """Layout whose flat indices address an oracle array (ns, *shape_halo) directly."""
+
+    def __init__(self, ns, shape):
+        n = [1] * (3 - len(shape)) + list(shape)
+        self.ns, self.n = ns, n
+        self.pitch = self.row_stride = n[2]
+        self.lead = 0
+        self.pop_stride = n[0] * n[1] * n[2]
+
+
+def _emulated_step(o, prod_methods, dom):
+    """The device's sequence on the host: ghosts poisoned, periodic-face slabs
+    filled (first ghost axis periodic, populations entering through every ghost
+    axis), the unwritten-ghost copy list, then the link lists."""
+    from pylbm_amd.boundary import unwritten_ghost_reads
+    from oracle.boundary import apply_links
+    F = o.F.copy()
+    ghost = np.ones(F.shape[1:], dtype=bool)
+    ghost[o._interior()[1:]] = False
+    F[:, ghost] = np.nan
+    shape = F.shape[1:]
+    h = o.h
+    labels = dom.box_label
+    for idx in zip(*np.nonzero(ghost)):
+        first = next(b for b in range(o.dim) if idx[b] < h[b] or idx[b] >= shape[b] - h[b])
+        side = 0 if idx[first] < h[first] else 1
+        if labels[2 * first + side] != -1:
+            continue
+        for k, vk in enumerate(o.v):
+            x = [idx[b] + vk[b] for b in range(o.dim)]
+            if all(h[b] <= x[b] < shape[b] - h[b] for b in range(o.dim)):
+                w = [int(idx[b]) + int(shape[b] - 2 * h[b]) * (int(idx[b] < h[b]) - int(idx[b] >= shape[b] - h[b]))
+                     for b in range(o.dim)]
+                F[(k,) + tuple(idx)] = F[(k,) + tuple(w)]
+    lay = _HostLayout(o.ns, shape)
+    st, ld = unwritten_ghost_reads(dom, dom.stencil, prod_methods, lay)
+    flat = F.reshape(-1)
+    flat[st] = flat[ld]
+    for meth in o.methods:
+        apply_links(F, meth)
+    return F, st.size
+
+
+@pytest.mark.parametrize("name", ["spheres", "lid", "d1q3"])
+def test_restricted_ghost_fill_equals_full_wrap(name):
+    from oracle.lbm import Simulation as O
+    d = {"spheres": lambda: cases.sphere_packing_d3q19(20, 14, 12, seed=3, solid_fraction=0.25),
+         "lid": lambda: cases.lid_cavity_d2q9(12),
+         "d1q3": lambda: cases.d1q3_advection(10, walls=False)}[name]()
+    o = O(d)
+    for _ in range(2):
+        o.one_time_step()
+    dom, pm = _product(d)
+    for m in pm:
+        m.fix_iload()
+    F_dev, ncopies = _emulated_step(o, pm, dom)
+    F_ref = o.F.copy()
+    o.ghost_fill(F_ref)
+    from oracle.boundary import apply_links
+    for meth in o.methods:
+        apply_links(F_ref, meth)
+    a, b = np.empty_like(F_ref), np.empty_like(F_ref)
+    o.pull_step(F_dev, a)
+    o.pull_step(F_ref, b)
+    inner = o._interior()
+    assert not np.isnan(a[inner]).any()
+    assert np.array_equal(a[inner], b[inner])
+    if name == "spheres":
+        assert ncopies > 0
