@@ -15,6 +15,9 @@ device sync.  Printed as one JSON line:
             counters on a plain copy kernel of known size, or null
   cpu_baseline  the oracle's C restatement of pylbm's generated Cython loop
             (FP64, AoS, -O3) on a bounded 128^3 sample, all host threads
+  surrogate the learned surrogate on the 512^2 periodic D2Q9 decaying flow:
+            trained on HIP lattice trajectories, then its simulated steps per
+            second against the plain FP32 lattice run and its velocity rel-L2
 
 ``--gpus N`` with N > 1 exits: this build runs on one GPU.
 """
@@ -121,6 +124,62 @@ def _cpu_baseline(target_s=8.0):
     return out, cores, n
 
 
+def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), K=16, judge_every=8,
+                   tol=1e-2):
+    """Train the surrogate on HIP lattice trajectories (512^2 periodic D2Q9, FP32, nu = 1e-3),
+    then advance held-out initial states by ``steps`` lattice steps with the plain run and with
+    the judged surrogate; simulated steps per second of both (wall clock, device-resident
+    state, one state read-back in both) and the surrogate's final velocity rel-L2."""
+    from pylbm_amd.surrogate import LatticeState, SurrogateRun, train_surrogate
+
+    lat = LatticeState(n=512, dtype="float32")
+    rt = lat.rt
+    net, norm, info = train_surrogate(lat, ntraj=ntraj, steps=steps, K=K, batch=8,
+                                      steps_per_traj=steps_per_traj)
+    nint = steps // K
+    run = SurrogateRun(net, lat, norm, K=K, judge_every=judge_every, tol=tol)
+    ref = rt.malloc(lat.npix * 16)
+    evals = []
+    for seed in eval_seeds:
+        s0 = lat.initial_state(seed)
+        rt.h2d(run.state, s0)
+        lat.restart(run.state)
+        lat.run(K)                       # warm-up of the plain path
+        lat.restart(run.state)
+        rt.sync()
+        t0 = time.perf_counter()
+        lat.run(K * nint)
+        lat.read_state(ref)
+        rt.sync()
+        t_lbm = time.perf_counter() - t0
+        rt.h2d(run.state, s0)
+        lat.restart(run.state)
+        rt.sync()
+        t0 = time.perf_counter()
+        run.start_from_lattice()
+        run.advance(nint)
+        rt.sync()
+        t_sur = time.perf_counter() - t0
+        err = run.rel_l2(run.state, ref)
+        evals.append({"seed": seed, "lbm_s": round(t_lbm, 5), "surrogate_s": round(t_sur, 5),
+                      "speedup": round(t_lbm / t_sur, 3), "rel_l2_velocity": float("%.3e" % err),
+                      "fallback_at_interval": run.fallback_at,
+                      "max_judge_drift": float("%.3e" % max(run.judges)) if run.judges else None})
+    rt.free(ref)
+    run.free()
+    net.free()
+    info.pop("losses", None)
+    return {
+        "workload": "512^2 periodic D2Q9 MRT decaying flow, FP32, nu=1e-3, random Fourier velocity "
+                    "E(k)~k^-5/3 on |k| in [4,16], rms 0.05",
+        "K": K, "judge_every": judge_every, "tol_rel_l2": tol, "simulated_steps": K * nint,
+        "speedup": round(min(e["speedup"] for e in evals), 3),
+        "rel_l2_velocity": max(e["rel_l2_velocity"] for e in evals),
+        "eval": evals,
+        "train": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +188,10 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sur-traj", type=int, default=200, help="surrogate training trajectories")
+    ap.add_argument("--sur-steps", type=int, default=2000, help="lattice steps per trajectory")
+    ap.add_argument("--sur-train", type=int, default=25, help="optimizer steps per trajectory")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="launch the plain copy kernel on a buffer of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -197,6 +260,10 @@ def main():
                "sample": "%d^3 D3Q19 MRT Taylor-Green, FP64 AoS, C restatement of pylbm's generated Cython "
                          "one_time_step + periodic ghost fill (oracle/cgen.py), %d steps on %d OpenMP threads, "
                          "%d steps on 1 thread" % (cn, res[cores][1], cores, res[1][1])}
+
+    if not args.no_surrogate:
+        del sim
+        extra["surrogate"] = _surrogate_leg(args.sur_traj, args.sur_steps, args.sur_train)
 
     out = {
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,

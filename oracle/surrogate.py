@@ -2,22 +2,54 @@
 
 The surrogate has no counterpart in pylbm.  This is the plain numpy statement
 of the network the HIP kernels (pylbm_amd/csrc/surrogate.hip) implement, used
-to check their outputs and gradients:
+to check their outputs and gradients.
 
-  x (N, H, W, 4) -> conv3x3(4 -> 32) -> SiLU -> conv3x3(32 -> 32) -> SiLU
-    -> conv3x3(32 -> 32) -> SiLU -> conv3x3(32 -> 3) -> increment (N, H, W, 3)
+Fine state (N, Hf, Wf, 4) = (rho, ux, uy, 0) is normalised per channel and
+unshuffled into 4x4 patches: coarse pixel (Y, X) channel (4 py + px) * 4 + ch
+holds fine pixel (4 Y + py, 4 X + px) channel ch.  Then
 
-with periodic padding, weights W[tap][ci][co] (tap = 3*(dy+1) + (dx+1)), biases
-b[co], loss = mean over all elements of (increment - target)^2, and Adam.
+  x (N, H, W, 64) -> conv3x3(64 -> C) -> SiLU -> conv3x3(C -> C) -> SiLU
+    -> conv3x3(C -> C) -> SiLU -> conv3x3(C -> 64) -> y (N, H, W, 64)
+
+with periodic padding, weights W[tap][ci][co] (tap = 3*(dy+1) + (dx+1)),
+biases b[co].  The loss is the mean of (y - target)^2 over the channels with
+ch = c % 4 < 3 (the slot ch = 3 carries nothing); y is the increment of the
+normalised state in units of the increment scale.
 
 Pinned by: central finite differences of the loss with respect to inputs,
-weights and biases (tests/test_oracle_surrogate.py), SiLU' against a finite
-difference, and a direct loop convolution on a tiny image.
+weights and biases, SiLU' against a finite difference, a direct loop
+convolution on a tiny image, and unshuffle/shuffle round trips
+(tests/test_oracle_surrogate.py).
 """
 import numpy as np
 
 TAPS = [(dy, dx) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
-CHANNELS = [4, 32, 32, 32, 3]
+PATCH = 4
+IN_CH = PATCH * PATCH * 4
+
+
+def channels(hidden=64):
+    return [IN_CH, hidden, hidden, hidden, IN_CH]
+
+
+def loss_mask():
+    """(64,) 1.0 on the channels that carry (rho, ux, uy), 0.0 on the empty slot."""
+    return (np.arange(IN_CH) % 4 < 3).astype(np.float64)
+
+
+def unshuffle(fine):
+    """(N, Hf, Wf, 4) -> (N, Hf/4, Wf/4, 64)."""
+    n, hf, wf, c = fine.shape
+    t = fine.reshape(n, hf // PATCH, PATCH, wf // PATCH, PATCH, c)
+    return t.transpose(0, 1, 3, 2, 4, 5).reshape(n, hf // PATCH, wf // PATCH, PATCH * PATCH * c)
+
+
+def shuffle(coarse):
+    """Inverse of :func:`unshuffle`."""
+    n, h, w, cc = coarse.shape
+    c = cc // (PATCH * PATCH)
+    t = coarse.reshape(n, h, w, PATCH, PATCH, c)
+    return t.transpose(0, 1, 3, 2, 4, 5).reshape(n, h * PATCH, w * PATCH, c)
 
 
 def silu(z):
@@ -55,11 +87,12 @@ def conv3x3_weight_grad(x, g):
     return dw, g.reshape(-1, g.shape[3]).sum(axis=0)
 
 
-def init_params(seed=0, scale_last=0.1):
+def init_params(seed=0, hidden=64, scale_last=0.1):
     rng = np.random.default_rng(seed)
+    ch = channels(hidden)
     params = []
     for li in range(4):
-        ci, co = CHANNELS[li], CHANNELS[li + 1]
+        ci, co = ch[li], ch[li + 1]
         std = np.sqrt(2.0 / (9 * ci))
         w = rng.normal(0.0, std, (9, ci, co))
         if li == 3:
@@ -69,7 +102,7 @@ def init_params(seed=0, scale_last=0.1):
 
 
 def forward(params, x):
-    """Returns (increment, cache)."""
+    """Returns (y, cache)."""
     acts, pre = [x], []
     a = x
     for li, (w, b) in enumerate(params):
@@ -81,10 +114,13 @@ def forward(params, x):
 
 
 def loss_and_grads(params, x, target):
+    """Masked mean-square loss over the fine (rho, ux, uy) values and its gradients."""
     y, (acts, pre) = forward(params, x)
-    diff = y - target
-    loss = float(np.mean(diff ** 2))
-    g = 2.0 * diff / diff.size
+    mask = loss_mask()
+    diff = (y - target) * mask
+    count = diff[..., 0].size * mask.sum()
+    loss = float(np.sum(diff ** 2) / count)
+    g = 2.0 * diff / count
     grads = [None] * 4
     for li in range(3, -1, -1):
         if li < 3:
@@ -94,6 +130,17 @@ def loss_and_grads(params, x, target):
         if li > 0:
             g = conv3x3_input_grad(g, params[li][0])
     return loss, grads
+
+
+def step_state(params, fine, shift, scale, dscale):
+    """One surrogate step of a fine state (N, Hf, Wf, 4): state + increment * dscale."""
+    sh = np.array(list(shift) + [0.0])
+    sc = np.array(list(scale) + [1.0])
+    ds = np.array(list(dscale) + [0.0])
+    x = (fine - sh) / sc
+    x[..., 3] = 0.0
+    y, _ = forward(params, unshuffle(x))
+    return fine + shuffle(y) * ds
 
 
 class Adam:

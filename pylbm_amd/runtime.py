@@ -92,6 +92,31 @@ CORE_SIGNATURES = {
                                 ctypes.POINTER(_D), _P], _I),
 }
 
+_F = ctypes.c_float
+_FP = ctypes.POINTER(ctypes.c_float)
+
+class LbmSurNet(ctypes.Structure):
+    _fields_ = [("act", ctypes.c_void_p * 4), ("wpk", ctypes.c_void_p * 4), ("bias", ctypes.c_void_p * 4),
+                ("hidden", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+
+
+SURROGATE_SIGNATURES = {
+    "lbm_sur_run": ([ctypes.POINTER(LbmSurNet), _P, _FP, _FP, _FP, _I64, _I, _I, _I, _P], _I),
+    "lbm_sur_conv": ([_P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P], _I),
+    "lbm_sur_apply_step": ([_P, _I, _P, _P, _P, _P, _FP, _FP, _FP, _I, _I, _I, _P], _I),
+    "lbm_sur_wgrad_blocks": ([], _I),
+    "lbm_sur_wgrad": ([_P, _I, _P, _I, _I, _I, _I, _P, _P, _P, _P], _I),
+    "lbm_sur_mse": ([_P, _P, _I64, _F, _P, _P, ctypes.POINTER(_D), _P], _I),
+    "lbm_sur_pack": ([_P, _I, _I, _I, _I, _P, _P, _P], _I),
+    "lbm_sur_adam": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _F, _F, _F, _I, _P], _I),
+    "lbm_sur_make_input": ([_P, _I, _I, _I, _FP, _FP, _P, _P], _I),
+    "lbm_sur_make_target": ([_P, _P, _I, _I, _I, _FP, _P, _P], _I),
+    "lbm_sur_moments_to_state": ([_P, ctypes.POINTER(LbmLayout), _I, _I, _I, _I, _I, _P, _P], _I),
+    "lbm_sur_state_to_moments": ([_P, _I, _I, ctypes.POINTER(LbmLayout), _I, _I, _I, _P, _P], _I),
+    "lbm_sur_rel_l2": ([_P, _P, _I64, _P, ctypes.POINTER(_D), _P], _I),
+    "lbm_sur_perturb": ([_P, _I64, _FP, ctypes.c_uint64, _P], _I),
+}
+
 GEN_SIGNATURES = {
     "lbm_gen_info": ([ctypes.POINTER(ctypes.c_int32)] * 3, _I),
     "lbm_gen_step": ([_P, _P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmParams), _I, _P], _I),
@@ -119,7 +144,8 @@ def load_core_library(path=None):
     path = path or lib_path()
     if not os.path.exists(path):
         raise HipUnavailable("%s is not built (run __graft_entry__.build())" % path)
-    return bind(ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL), CORE_SIGNATURES)
+    lib = bind(ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL), CORE_SIGNATURES)
+    return bind(lib, SURROGATE_SIGNATURES)
 
 
 class Runtime:

@@ -1,0 +1,560 @@
+"""Learned surrogate of a periodic D2Q9 run (no counterpart in pylbm).
+
+A 4-layer periodic 3x3 convolution network on 4x4 patches maps the normalised
+moment fields (rho, ux, uy) at t to their increment over K lattice steps.  The
+fine state is unshuffled into coarse pixels of 64 channels (16 fine pixels x 4
+fields), so the convolutions run on a grid 16 times smaller and see 16 fine
+cells in every direction (farther than sound travels in K = 16 steps); the
+network is 64 -> C -> C -> C -> 64 channels with SiLU, C = 64 by default.
+Training and inference run on the project's own HIP kernels (FP16 MFMA
+operands, FP32 accumulation, FP32 master weights, Adam); training data are
+trajectories the HIP lattice Boltzmann run itself produces.
+
+The moments are read through the same route as ``Simulation.m``
+(pylbm/simulation.py:215-224: f2m over the halo domain) and the lattice
+Boltzmann run is restarted from a surrogate state with pylbm's moment
+initialisation, conserved moments -> equilibrium -> m2f
+(pylbm/simulation.py:257-320).
+
+:class:`SurrogateRun` advances a simulation by intervals of K steps: the
+network does every interval except every ``judge_every``-th one, which the
+plain lattice Boltzmann run computes from the current surrogate state; the
+rel-L2 distance of the velocity between that interval's lattice Boltzmann
+result and the network's prediction for it is the drift.  Past ``tol`` (or on
+a non-finite drift) the run falls back to the plain lattice Boltzmann method
+for the rest of the advance.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import runtime as rtm
+from .cases import QX, QY, RHO, decaying_d2q9, fourier_velocity
+from .simulation import Simulation
+
+PATCH = 4
+IN_CH = 64
+EPI_ACT, EPI_LIN, EPI_BWD = 0, 1, 2
+
+
+def channels(hidden=64):
+    if hidden not in (32, 64):
+        raise ValueError("hidden channels must be 32 or 64 (got %r)" % hidden)
+    return (IN_CH, hidden, hidden, hidden, IN_CH)
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def init_params(seed=0, hidden=64, scale_last=0.1):
+    """He-normal weights w[9][cin][cout] (last layer scaled down), zero biases."""
+    rng = np.random.default_rng(seed)
+    ch = channels(hidden)
+    out = []
+    for li in range(4):
+        std = np.sqrt(2.0 / (9 * ch[li]))
+        w = rng.normal(0.0, std, (9, ch[li], ch[li + 1]))
+        if li == 3:
+            w *= scale_last
+        out.append([w.astype(np.float32), np.zeros(ch[li + 1], np.float32)])
+    return out
+
+
+class SurrogateNet:
+    """Device-resident network for batches of up to ``batch`` fine fields of Hf x Wf."""
+
+    def __init__(self, rt, Hf, Wf, batch=1, hidden=64, train=True, params=None, seed=0):
+        if Hf % PATCH or Wf % (PATCH * 32):
+            raise ValueError("the surrogate needs Hf % 4 == 0 and Wf % 128 == 0 (got %dx%d)" % (Hf, Wf))
+        self.rt, self.lib, self.batch, self.train = rt, rt.lib, batch, train
+        self.Hf, self.Wf = Hf, Wf
+        self.H, self.W = Hf // PATCH, Wf // PATCH
+        self.hidden = hidden
+        self.ch = channels(hidden)
+        ch = self.ch
+        npix = batch * self.H * self.W
+        self._bufs = []
+        al = self._alloc
+        self.xin = al(npix * IN_CH * 2)
+        self.act = [self.xin] + [al(npix * hidden * 2) for _ in range(3)]
+        self.y = al(npix * IN_CH * 4)
+        self.w = [al(9 * ch[i] * ch[i + 1] * 4) for i in range(4)]
+        self.b = [al(ch[i + 1] * 4) for i in range(4)]
+        self.afwd = [al(9 * ch[i] * ch[i + 1] * 2) for i in range(4)]
+        if train:
+            self.z = [al(npix * hidden * 2) for _ in range(3)]
+            self.tgt = al(npix * IN_CH * 4)
+            self.dy = al(npix * IN_CH * 2)
+            self.g = [al(npix * hidden * 2) for _ in range(2)]
+            self.abwd = [None] + [al(9 * ch[i] * ch[i + 1] * 2) for i in range(1, 4)]
+            nblk = self.lib.lbm_sur_wgrad_blocks()
+            self.partial = al(nblk * (9 * 64 * 64 + 64) * 4)
+            self.dwt = [al(9 * ch[i] * ch[i + 1] * 4) for i in range(4)]
+            self.db = [al(ch[i + 1] * 4) for i in range(4)]
+            nparam = [9 * ch[i] * ch[i + 1] + ch[i + 1] for i in range(4)]
+            self.m = [al(n * 4) for n in nparam]
+            self.v = [al(n * 4) for n in nparam]
+            for i in range(4):
+                self._zero(self.m[i], nparam[i] * 4)
+                self._zero(self.v[i], nparam[i] * 4)
+            self.lpart = al(256 * 8)
+            self.step = 0
+        self.set_params(params if params is not None else init_params(seed, hidden))
+
+    # ------------------------------------------------------------ memory
+    def _alloc(self, nbytes):
+        p = self.rt.malloc(max(int(nbytes), 256))
+        self._bufs.append(p)
+        return p
+
+    def _zero(self, ptr, nbytes):
+        self.rt.check(self.lib.lbm_memset(ctypes.c_void_p(ptr), 0, int(nbytes), self.rt.stream), "memset")
+
+    def free(self):
+        for p in self._bufs:
+            self.rt.free(p)
+        self._bufs = []
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ params
+    def set_params(self, params):
+        ch = self.ch
+        for i, (w, b) in enumerate(params):
+            w = np.ascontiguousarray(w, dtype=np.float32).reshape(9, ch[i], ch[i + 1])
+            b = np.ascontiguousarray(b, dtype=np.float32).reshape(ch[i + 1])
+            self.rt.h2d(self.w[i], w)
+            self.rt.h2d(self.b[i], b)
+        self.pack()
+
+    def get_params(self):
+        ch = self.ch
+        out = []
+        for i in range(4):
+            w = np.empty((9, ch[i], ch[i + 1]), np.float32)
+            b = np.empty(ch[i + 1], np.float32)
+            self.rt.d2h(w, self.w[i])
+            self.rt.d2h(b, self.b[i])
+            out.append([w, b])
+        self.rt.sync()
+        return out
+
+    def pack(self):
+        ch = self.ch
+        for i in range(4):
+            abwd = self.abwd[i] if self.train else None
+            self.rt.check(self.lib.lbm_sur_pack(ctypes.c_void_p(self.w[i]), ch[i], ch[i + 1], ch[i], ch[i + 1],
+                                                ctypes.c_void_p(self.afwd[i]), ctypes.c_void_p(abwd),
+                                                self.rt.stream), "lbm_sur_pack")
+
+    # ------------------------------------------------------------ passes
+    def _conv(self, src, cinp, wpk, bias, zin, out, zout, coutp, epi, n):
+        self.rt.check(self.lib.lbm_sur_conv(ctypes.c_void_p(src), cinp, ctypes.c_void_p(wpk),
+                                            ctypes.c_void_p(bias), ctypes.c_void_p(zin), ctypes.c_void_p(out),
+                                            ctypes.c_void_p(zout), coutp, epi, n, self.H, self.W,
+                                            self.rt.stream), "lbm_sur_conv")
+
+    def hidden_layers(self, n=1, keep=False):
+        """Layers 0..2 from xin; keep=True stores the pre-activations for backward."""
+        if n > self.batch:
+            raise ValueError("batch %d exceeds the allocated %d" % (n, self.batch))
+        if keep and not self.train:
+            raise ValueError("network allocated without training buffers")
+        ch = self.ch
+        for i in range(3):
+            self._conv(self.act[i], ch[i], self.afwd[i], self.b[i], None, self.act[i + 1],
+                       self.z[i] if keep else None, ch[i + 1], EPI_ACT, n)
+
+    def forward(self, n=1, keep=False):
+        """y (FP32 coarse [n][H][W][64]) from xin."""
+        self.hidden_layers(n, keep)
+        self._conv(self.act[3], self.hidden, self.afwd[3], self.b[3], None, self.y, None, IN_CH, EPI_LIN, n)
+        return self.y
+
+    def apply(self, d_state, norm_args, n=1):
+        """Inference step: d_state += y * dscale and xin = the next input (one fused kernel)."""
+        dscale, shift, scale = norm_args
+        self.hidden_layers(n)
+        self.rt.check(self.lib.lbm_sur_apply_step(ctypes.c_void_p(self.act[3]), self.hidden,
+                                                  ctypes.c_void_p(self.afwd[3]), ctypes.c_void_p(self.b[3]),
+                                                  ctypes.c_void_p(d_state), ctypes.c_void_p(self.xin), dscale,
+                                                  shift, scale, n, self.H, self.W, self.rt.stream),
+                      "lbm_sur_apply_step")
+
+    def run(self, d_state, norm_args, nsteps):
+        """nsteps inference steps of one field in one library call; xin must hold d_state's input."""
+        dscale, shift, scale = norm_args
+        if not hasattr(self, "_cnet"):
+            c = rtm.LbmSurNet()
+            for i in range(4):
+                c.act[i] = self.act[i]
+                c.wpk[i] = self.afwd[i]
+                c.bias[i] = self.b[i]
+            c.hidden = self.hidden
+            self._cnet = c
+        self.rt.check(self.lib.lbm_sur_run(ctypes.byref(self._cnet), ctypes.c_void_p(d_state), dscale, shift, scale,
+                                           int(nsteps), 1, self.H, self.W, self.rt.stream), "lbm_sur_run")
+
+    def loss(self, n, gscale=1.0, sync=True):
+        """Masked MSE of y against tgt; writes dY = 2 gscale (y - t). Returns the loss if sync."""
+        out = ctypes.c_double(0.0)
+        self.rt.check(self.lib.lbm_sur_mse(ctypes.c_void_p(self.y), ctypes.c_void_p(self.tgt), n * self.H * self.W,
+                                           float(gscale), ctypes.c_void_p(self.dy), ctypes.c_void_p(self.lpart),
+                                           ctypes.byref(out) if sync else None, self.rt.stream), "lbm_sur_mse")
+        return out.value if sync else None
+
+    def backward(self, n):
+        """Weight and bias gradients (dwt, db) of every layer from dY."""
+        lib, st, ch = self.lib, self.rt.stream, self.ch
+        grad_in = self.dy
+        for i in range(3, -1, -1):
+            self.rt.check(lib.lbm_sur_wgrad(ctypes.c_void_p(self.act[i]), ch[i], ctypes.c_void_p(grad_in), ch[i + 1],
+                                            n, self.H, self.W, ctypes.c_void_p(self.partial),
+                                            ctypes.c_void_p(self.dwt[i]), ctypes.c_void_p(self.db[i]), st),
+                          "lbm_sur_wgrad")
+            if i > 0:
+                out = self.g[i % 2]
+                self._conv(grad_in, ch[i + 1], self.abwd[i], None, self.z[i - 1], out, None, ch[i], EPI_BWD, n)
+                grad_in = out
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, eps_scale=1.0):
+        """One Adam step on every layer, then repack the FP16 operands."""
+        self.step += 1
+        ch = self.ch
+        for i in range(4):
+            self.rt.check(self.lib.lbm_sur_adam(ctypes.c_void_p(self.w[i]), ctypes.c_void_p(self.b[i]),
+                                                ctypes.c_void_p(self.m[i]), ctypes.c_void_p(self.v[i]),
+                                                ctypes.c_void_p(self.dwt[i]), ctypes.c_void_p(self.db[i]),
+                                                ch[i], ch[i + 1], ch[i], ch[i + 1], float(lr), float(b1), float(b2),
+                                                float(eps * eps_scale), self.step, self.rt.stream), "lbm_sur_adam")
+        self.pack()
+
+    def train_step(self, n, lr, sync=False):
+        """forward, loss-scaled MSE gradient, backward, Adam (same step as the unscaled loss)."""
+        self.forward(n, keep=True)
+        loss = self.loss(n, gscale=1.0, sync=sync)
+        self.backward(n)
+        self.adam(lr=lr, eps_scale=float(n * self.H * self.W * 48))
+        return loss
+
+    def grads(self):
+        """Host copies of (dW[9][cin][cout], db[cout]) per layer, as produced by backward."""
+        ch = self.ch
+        out = []
+        for i in range(4):
+            dwt = np.empty((9, ch[i + 1], ch[i]), np.float32)
+            db = np.empty(ch[i + 1], np.float32)
+            self.rt.d2h(dwt, self.dwt[i])
+            self.rt.d2h(db, self.db[i])
+            self.rt.sync()
+            out.append([dwt.transpose(0, 2, 1).astype(np.float64), db.astype(np.float64)])
+        return out
+
+
+class Normalization:
+    """Per-channel input shift/scale and increment scale of the network."""
+
+    def __init__(self, shift, scale, dscale):
+        self.shift = np.asarray(shift, np.float64)
+        self.scale = np.asarray(scale, np.float64)
+        self.dscale = np.asarray(dscale, np.float64)
+
+    @classmethod
+    def from_states(cls, states, rho0=1.0):
+        """states: host array (T, H, W, 4) of one trajectory at interval spacing."""
+        s = np.asarray(states, np.float64)
+        d = s[1:] - s[:-1]
+        drho = np.sqrt(np.mean((s[..., 0] - rho0) ** 2))
+        u = np.sqrt(np.mean(s[..., 1] ** 2 + s[..., 2] ** 2) / 2.0)
+        ddrho = np.sqrt(np.mean(d[..., 0] ** 2))
+        du = np.sqrt(np.mean(d[..., 1] ** 2 + d[..., 2] ** 2) / 2.0)
+        return cls((rho0, 0.0, 0.0), (max(drho, 1e-12), u, u), (max(ddrho, 1e-12), du, du))
+
+
+class LatticeState:
+    """One periodic D2Q9 simulation plus the moment <-> state couplings the surrogate needs."""
+
+    def __init__(self, n=512, nu=1e-3, dtype="float32", seed=0, rt=None):
+        self.sim = Simulation(decaying_d2q9(n, nu=nu, seed=seed), dtype=dtype)
+        if not self.sim.periodic_only:
+            raise ValueError("the surrogate handles fully periodic runs only")
+        self.rt = self.sim.rt
+        self.lib = self.rt.lib
+        self.H, self.W = self.sim.domain.shape_in
+        c = self.sim.scheme.consm
+        self.irho, self.iqx, self.iqy = c[RHO], c[QX], c[QY]
+        self.npix = self.H * self.W
+
+    def read_state(self, d_state):
+        """Conserved moments of the current populations into an FP32 state buffer."""
+        sim = self.sim
+        sim._f2m(sim.container.F, sim.container.m)
+        sim._update_m = False
+        m = sim.container.m
+        self.rt.check(self.lib.lbm_sur_moments_to_state(ctypes.c_void_p(m.ptr), ctypes.byref(m.layout), self.irho,
+                                                        self.iqx, self.iqy, self.H, self.W, ctypes.c_void_p(d_state),
+                                                        self.rt.stream), "lbm_sur_moments_to_state")
+
+    def restart(self, d_state):
+        """Populations from a state: conserved moments -> equilibrium -> m2f (pylbm moment init)."""
+        sim = self.sim
+        m = sim.container.m
+        self.rt.check(self.lib.lbm_sur_state_to_moments(ctypes.c_void_p(d_state), self.H, self.W,
+                                                        ctypes.byref(m.layout), self.irho, self.iqx, self.iqy,
+                                                        ctypes.c_void_p(m.ptr), self.rt.stream),
+                      "lbm_sur_state_to_moments")
+        sim._equilibrium(m)
+        sim._m2f(m, sim.container.F)
+        sim.container.Fnew.copy_from(sim.container.F)
+        sim._update_m = True
+
+    def run(self, nsteps):
+        self.sim.run(nsteps)
+
+    def initial_state(self, seed, rms=0.05, kmin=4.0, kmax=16.0):
+        """Host FP32 state of the random Fourier velocity of ``seed`` (rho = 1)."""
+        ux, uy = fourier_velocity(self.H, kmin=kmin, kmax=kmax, rms=rms, seed=seed)
+        s = np.zeros((self.H, self.W, 4), np.float32)
+        s[..., 0] = 1.0
+        s[..., 1] = ux
+        s[..., 2] = uy
+        return s
+
+
+class Trajectories:
+    """Device-resident pool of trajectories (states every K steps) for training."""
+
+    def __init__(self, lattice, K=16, steps=2000, pool=16, kband=(4.0, 16.0)):
+        self.lat, self.K, self.kband = lattice, K, kband
+        self.T = steps // K + 1
+        self.pool = pool
+        self.stride = lattice.npix * 16
+        self.base = lattice.rt.malloc(pool * self.T * self.stride)
+        self.filled = 0
+        self.next = 0
+
+    def state_ptr(self, slot, t):
+        return self.base + (slot * self.T + t) * self.stride
+
+    def generate(self, seed):
+        """Run one trajectory from the Fourier state of ``seed``; returns its pool slot."""
+        lat, rt = self.lat, self.lat.rt
+        slot = self.next
+        s0 = self.state_ptr(slot, 0)
+        rt.h2d(s0, lat.initial_state(seed, kmin=self.kband[0], kmax=self.kband[1]))
+        lat.restart(s0)
+        lat.read_state(s0)
+        for t in range(1, self.T):
+            lat.run(self.K)
+            lat.read_state(self.state_ptr(slot, t))
+        self.next = (self.next + 1) % self.pool
+        self.filled = min(self.filled + 1, self.pool)
+        return slot
+
+    def download(self, slot, every=1):
+        out = []
+        for t in range(0, self.T, every):
+            h = np.empty((self.lat.H, self.lat.W, 4), np.float32)
+            self.lat.rt.d2h(h, self.state_ptr(slot, t))
+            out.append(h)
+        self.lat.rt.sync()
+        return np.stack(out)
+
+    def free(self):
+        if self.base:
+            self.lat.rt.free(self.base)
+            self.base = None
+
+
+class Trainer:
+    """Adam training of a SurrogateNet on pairs (state_t, state_{t+K}) of a pool.
+
+    pushforward: probability that a sample's input is the network's own one-step
+    prediction from state_{t-K} (no gradient through it) instead of state_t, so
+    the network learns on the distribution its rollouts produce; noise: standard
+    deviation of Gaussian noise added to the input state, in units of the input
+    scale (the target is the increment from the perturbed state)."""
+
+    def __init__(self, net, data, norm, seed=0, pushforward=0.0, noise=0.0):
+        self.net, self.data, self.norm = net, data, norm
+        self.rng = np.random.default_rng(seed)
+        self.lib, self.rt = net.lib, net.rt
+        self._shift, self._scale, self._dscale = _f3(norm.shift), _f3(norm.scale), _f3(norm.dscale)
+        self._sigma = _f3(np.asarray(norm.scale) * noise)
+        self.pushforward, self.noise = pushforward, noise
+        self.scratch = None
+        if pushforward > 0 or noise > 0:
+            self.scratch = net.rt.malloc(net.batch * data.stride)
+        self.losses = []
+        self._draw = 0
+
+    def free(self):
+        if self.scratch:
+            self.rt.free(self.scratch)
+            self.scratch = None
+
+    def _load_batch(self, n):
+        net, d, lat = self.net, self.data, self.data.lat
+        npix = lat.npix
+        norm_args = (self._dscale, self._shift, self._scale)
+        picks = []
+        for b in range(n):
+            slot = int(self.rng.integers(d.filled))
+            pf = self.pushforward > 0 and self.rng.random() < self.pushforward
+            t = int(self.rng.integers(1 if pf else 0, d.T - 1))
+            picks.append((slot, t, pf))
+        # pushforward inputs first: they run the network on the batch-0 buffers
+        for b, (slot, t, pf) in enumerate(picks):
+            sb = self.scratch + b * d.stride if self.scratch else None
+            if pf:
+                self.rt.d2d(sb, d.state_ptr(slot, t - 1), d.stride)
+                self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(sb), 1, lat.H, lat.W, self._shift,
+                                                          self._scale, ctypes.c_void_p(net.xin), self.rt.stream),
+                              "lbm_sur_make_input")
+                net.run(sb, norm_args, 1)
+            elif self.noise > 0:
+                self.rt.d2d(sb, d.state_ptr(slot, t), d.stride)
+            if self.noise > 0:
+                self._draw += 1
+                self.rt.check(self.lib.lbm_sur_perturb(ctypes.c_void_p(sb), npix, self._sigma,
+                                                       int(self._draw * 1000003 + b), self.rt.stream),
+                              "lbm_sur_perturb")
+        for b, (slot, t, pf) in enumerate(picks):
+            s0 = self.scratch + b * d.stride if (pf or self.noise > 0) else d.state_ptr(slot, t)
+            s1 = d.state_ptr(slot, t + 1)
+            self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(s0), 1, lat.H, lat.W, self._shift,
+                                                      self._scale, ctypes.c_void_p(net.xin + b * npix * 8),
+                                                      self.rt.stream), "lbm_sur_make_input")
+            self.rt.check(self.lib.lbm_sur_make_target(ctypes.c_void_p(s0), ctypes.c_void_p(s1), 1, lat.H, lat.W,
+                                                       self._dscale, ctypes.c_void_p(net.tgt + b * npix * 16),
+                                                       self.rt.stream), "lbm_sur_make_target")
+
+    def steps(self, nsteps, lr, batch=None, log_every=0):
+        n = batch or self.net.batch
+        for i in range(nsteps):
+            self._load_batch(n)
+            sync = bool(log_every) and (i % log_every == 0 or i == nsteps - 1)
+            loss = self.net.train_step(n, lr(self.net.step) if callable(lr) else lr, sync=sync)
+            if sync:
+                self.losses.append((self.net.step, loss))
+
+
+class SurrogateRun:
+    """Advance a LatticeState by intervals of K steps with the network, judged by the plain run."""
+
+    def __init__(self, net, lattice, norm, K=16, judge_every=8, tol=1e-2):
+        if net.Hf != lattice.H or net.Wf != lattice.W:
+            raise ValueError("network and lattice sizes differ")
+        self.net, self.lat, self.norm = net, lattice, norm
+        self.K, self.judge_every, self.tol = K, judge_every, tol
+        self.rt, self.lib = lattice.rt, lattice.lib
+        self.state = self.rt.malloc(lattice.npix * 16)
+        self.pred = self.rt.malloc(lattice.npix * 16)
+        self.lpart = self.rt.malloc(512 * 8)
+        self._shift, self._scale, self._dscale = _f3(norm.shift), _f3(norm.scale), _f3(norm.dscale)
+        self.judges = []
+        self.fallback_at = None
+
+    def free(self):
+        for p in (self.state, self.pred, self.lpart):
+            self.rt.free(p)
+        self.state = self.pred = self.lpart = None
+
+    def _make_input(self, d_state):
+        self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(d_state), 1, self.lat.H, self.lat.W, self._shift,
+                                                  self._scale, ctypes.c_void_p(self.net.xin), self.rt.stream),
+                      "lbm_sur_make_input")
+
+    def _net_step(self, d_state):
+        """d_state += increment(net(xin)); xin must hold d_state's input and then holds the next one."""
+        self.net.apply(d_state, (self._dscale, self._shift, self._scale))
+
+    def rel_l2(self, a, b):
+        out = ctypes.c_double(0.0)
+        self.rt.check(self.lib.lbm_sur_rel_l2(ctypes.c_void_p(a), ctypes.c_void_p(b), self.lat.npix,
+                                              ctypes.c_void_p(self.lpart), ctypes.byref(out), self.rt.stream),
+                      "lbm_sur_rel_l2")
+        return out.value
+
+    def start_from_lattice(self):
+        self.lat.read_state(self.state)
+        self.judges = []
+        self.fallback_at = None
+
+    def advance(self, nintervals):
+        """nintervals intervals of K lattice steps; state ends in self.state (and in the lattice)."""
+        lat, K, je = self.lat, self.K, self.judge_every
+        norm_args = (self._dscale, self._shift, self._scale)
+        self._make_input(self.state)
+        i = 0
+        while i < nintervals:
+            if self.fallback_at is not None:
+                lat.run(K * (nintervals - i))
+                lat.read_state(self.state)
+                return
+            nsur = min(je - 1 - (i % je), nintervals - i)
+            if nsur > 0:
+                self.net.run(self.state, norm_args, nsur)
+                i += nsur
+                continue
+            # the plain run computes this interval from the surrogate state
+            self.rt.d2d(self.pred, self.state, lat.npix * 16)
+            self.net.run(self.pred, norm_args, 1)
+            lat.restart(self.state)
+            lat.run(K)
+            lat.read_state(self.state)
+            drift = self.rel_l2(self.pred, self.state)
+            self.judges.append(drift)
+            if not drift <= self.tol:  # NaN drift falls back too
+                self.fallback_at = i
+            else:
+                self._make_input(self.state)
+            i += 1
+        lat.restart(self.state)
+
+
+def train_surrogate(lattice, ntraj=200, steps=2000, K=16, pool=16, batch=8, steps_per_traj=25, lr=1e-3,
+                    lr_min=1e-5, seed=0, kband=(4.0, 16.0), hidden=64, pushforward=0.0, noise=0.0, log=None):
+    """Generate ntraj trajectories with the HIP lattice Boltzmann run and train a network on them.
+
+    Returns (net, norm, info). Trajectory seeds are seed * 100000 + j."""
+    rt = lattice.rt
+    data = Trajectories(lattice, K=K, steps=steps, pool=pool, kband=kband)
+    t0 = time.perf_counter()
+    slot = data.generate(seed * 100000)
+    norm = Normalization.from_states(data.download(slot, every=4))
+    net = SurrogateNet(rt, lattice.H, lattice.W, batch=batch, hidden=hidden, train=True, seed=seed)
+    trainer = Trainer(net, data, norm, seed=seed, pushforward=pushforward, noise=noise)
+    total = ntraj * steps_per_traj
+
+    def sched(step):
+        f = min(step / float(total), 1.0)
+        return lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * f))
+
+    t_gen = time.perf_counter() - t0
+    t_train = 0.0
+    for j in range(ntraj):
+        if j > 0:
+            ta = time.perf_counter()
+            data.generate(seed * 100000 + j)
+            rt.sync()
+            t_gen += time.perf_counter() - ta
+        ta = time.perf_counter()
+        trainer.steps(steps_per_traj, sched, batch=batch, log_every=steps_per_traj)
+        rt.sync()
+        t_train += time.perf_counter() - ta
+        if log and (j % 20 == 0 or j == ntraj - 1):
+            log("traj %d/%d loss %.4e (gen %.1fs train %.1fs)" % (j + 1, ntraj, trainer.losses[-1][1], t_gen, t_train))
+    data.free()
+    trainer.free()
+    info = {"trajectories": ntraj, "steps_per_trajectory": steps, "K": K, "optimizer_steps": net.step,
+            "batch": batch, "hidden": hidden, "pushforward": pushforward, "noise": noise,
+            "final_loss": trainer.losses[-1][1], "generate_s": t_gen, "train_s": t_train,
+            "losses": trainer.losses}
+    return net, norm, info

@@ -1,5 +1,5 @@
 """The C-ABI libraries load and export every entry point include/pylbm_hip.h
-declares (no GPU needed; nothing is called)."""
+and include/pylbm_surrogate.h declare (no GPU needed; nothing is called)."""
 import ctypes
 import os
 import re
@@ -8,10 +8,11 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pylbm_hip.h")
+SUR_HEADER = os.path.join(ROOT, "include", "pylbm_surrogate.h")
 
 
-def _declared():
-    txt = open(HEADER).read()
+def _declared(header=HEADER):
+    txt = open(header).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(lbm_\w+)\s*\(", txt, flags=re.M)
     return sorted(set(names))
@@ -29,6 +30,16 @@ def test_core_exports():
     for n in core:
         assert hasattr(lib, n), n
     assert set(core) == set(CORE_SIGNATURES)
+
+
+def test_surrogate_exports():
+    from pylbm_amd.runtime import SURROGATE_SIGNATURES, load_core_library
+    lib = load_core_library()
+    names = _declared(SUR_HEADER)
+    assert "lbm_sur_conv" in names and "lbm_sur_wgrad" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(SURROGATE_SIGNATURES)
 
 
 def test_generated_exports():

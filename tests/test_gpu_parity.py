@@ -175,3 +175,25 @@ def test_layouts_bit_identical(hip, dtype, make):
     a.run(6)
     b.run(6)
     assert np.array_equal(a.container.F.fields_raw(), b.container.F.fields_raw())
+
+
+# ------------------------------------------------------- full-size configurations
+def test_lid_cavity_full_size_fp64_parity(hip):
+    """The 128^2 lid-driven cavity at Re = 100 for its full 2000 steps."""
+    d = cases.lid_cavity_d2q9(128)
+    sim, ref = Simulation(d), Oracle(d)
+    sim.run(2000)
+    for _ in range(2000):
+        ref.one_time_step()
+    _compare(sim, ref, [cases.RHO, cases.QX, cases.QY], TOL64)
+
+
+def test_karman_full_size_fp64_parity(hip):
+    """The 1024 x 256 channel with the r = 20 cylinder (ABB inflow, NeumannX outflow,
+    bounce-back walls and cylinder), 300 steps (the oracle's share of the test time)."""
+    d = cases.karman_d2q9()
+    sim, ref = Simulation(d), Oracle(d)
+    sim.run(300)
+    for _ in range(300):
+        ref.one_time_step()
+    _compare(sim, ref, [cases.RHO, cases.QX, cases.QY], TOL64)

@@ -1,0 +1,334 @@
+"""GPU tests of the learned surrogate: every kernel against the numpy oracle
+(oracle/surrogate.py, itself pinned by finite differences), the lattice
+coupling against the Simulation's own moment path, and what training buys.
+
+Tolerances: the kernels round every stored activation, gradient and packed
+weight to FP16 (unit roundoff 2^-11 = 4.9e-4) and accumulate in FP32.  A
+single convolution fed the device's own FP16 input and FP16 weights therefore
+matches the FP64 oracle to one FP16 rounding of its output; whole passes are
+held to a relative L2 error of 5e-3 (forward) and 2e-2 (gradients, which go
+through three more FP16 roundings per layer).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel_err  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+H16 = np.float16
+
+
+def _rl2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from pylbm_amd import runtime
+    return runtime.get()
+
+
+def _round16(params):
+    return [[w.astype(H16).astype(np.float64), b.astype(np.float64)] for w, b in params]
+
+
+def _download(rt, ptr, shape, dtype):
+    h = np.empty(shape, dtype)
+    rt.d2h(h, ptr)
+    rt.sync()
+    return h
+
+
+def _net(rt, N, Hf, Wf, hidden=64, seed=0, last=0.1):
+    from pylbm_amd.surrogate import SurrogateNet, init_params
+    params = init_params(seed, hidden=hidden, scale_last=last)
+    # non-zero biases exercise the bias paths
+    rng = np.random.default_rng(seed + 1)
+    for w, b in params:
+        b[...] = rng.normal(0.0, 0.1, b.shape)
+    return SurrogateNet(rt, Hf, Wf, batch=N, hidden=hidden, train=True, params=params), params
+
+
+def _coarse_input(rng, N, H, W):
+    x = rng.normal(0.0, 1.0, (N, H, W, 64))
+    x[..., 3::4] = 0.0
+    return x.astype(H16).astype(np.float64)
+
+
+def _upload_input(rt, net, x):
+    rt.h2d(net.xin, x.astype(H16))
+    return x
+
+
+SHAPES = [(2, 32, 128, 64), (1, 12, 256, 64), (1, 4, 128, 32), (2, 8, 128, 32)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_forward_layers_match_oracle(rt, shape):
+    """Each convolution against the oracle fed the device's own FP16 input."""
+    from oracle import surrogate as osur
+    N, Hf, Wf, C = shape
+    net, params = _net(rt, N, Hf, Wf, hidden=C)
+    H, W = net.H, net.W
+    x = _upload_input(rt, net, _coarse_input(np.random.default_rng(3), N, H, W))
+    net.forward(N, keep=True)
+    p16 = _round16(params)
+    a = x
+    for li in range(3):
+        z_dev = _download(rt, net.z[li], (N, H, W, C), H16).astype(np.float64)
+        a_dev = _download(rt, net.act[li + 1], (N, H, W, C), H16).astype(np.float64)
+        z_ref = osur.conv3x3(a, p16[li][0], p16[li][1])
+        tol = 2.0 ** -11 * np.abs(z_ref) + 1e-5 * np.max(np.abs(z_ref))
+        assert np.all(np.abs(z_dev - z_ref) <= tol), "layer %d pre-activation" % li
+        # a = silu(z) of the FP32 z before its rounding: |silu'| <= 1.1 carries z's rounding
+        s_ref = osur.silu(z_dev)
+        tol = 2.0 ** -11 * (np.abs(s_ref) + 1.2 * np.abs(z_dev)) + 1e-6
+        assert np.all(np.abs(a_dev - s_ref) <= tol), "layer %d SiLU" % li
+        a = a_dev
+    y_dev = _download(rt, net.y, (N, H, W, 64), np.float32).astype(np.float64)
+    y_ref = osur.conv3x3(a, p16[3][0], p16[3][1])
+    assert _rl2(y_dev, y_ref) <= 1e-5
+    net.free()
+
+
+def test_forward_matches_oracle_end_to_end(rt):
+    from oracle import surrogate as osur
+    N, Hf, Wf = 2, 64, 128
+    net, params = _net(rt, N, Hf, Wf, seed=5, last=1.0)
+    x = _upload_input(rt, net, _coarse_input(np.random.default_rng(7), N, net.H, net.W))
+    net.forward(N)
+    y_dev = _download(rt, net.y, (N, net.H, net.W, 64), np.float32)
+    p = [[w.astype(np.float64), b.astype(np.float64)] for w, b in params]
+    y_ref, _ = osur.forward(p, x)
+    assert _rl2(y_dev, y_ref) <= 5e-3
+    net.free()
+
+
+def test_make_input_target_and_apply_step(rt):
+    """Unshuffle of state and target, and the fused inference step, against the oracle."""
+    from oracle import surrogate as osur
+    N, Hf, Wf = 1, 16, 128
+    net, params = _net(rt, N, Hf, Wf, seed=8, last=1.0)
+    rng = np.random.default_rng(9)
+    s0 = np.zeros((N, Hf, Wf, 4), np.float32)
+    s0[..., 0] = 1.0 + 1e-3 * rng.normal(size=(N, Hf, Wf))
+    s0[..., 1:3] = 0.05 * rng.normal(size=(N, Hf, Wf, 2))
+    s1 = s0 + np.float32(1e-3) * rng.normal(size=s0.shape).astype(np.float32)
+    s1[..., 3] = 0.0
+    shift, scale, dscale = (1.0, 0.0, 0.0), (2e-3, 0.05, 0.05), (1e-3, 2e-3, 2e-3)
+    f3 = lambda v: (ctypes.c_float * 3)(*v)  # noqa: E731
+    d0, d1, dt = rt.malloc(s0.nbytes), rt.malloc(s1.nbytes), rt.malloc(s0.nbytes * 4)
+    rt.h2d(d0, s0)
+    rt.h2d(d1, s1)
+    rt.check(rt.lib.lbm_sur_make_input(ctypes.c_void_p(d0), N, Hf, Wf, f3(shift), f3(scale),
+                                       ctypes.c_void_p(net.xin), rt.stream), "make_input")
+    rt.check(rt.lib.lbm_sur_make_target(ctypes.c_void_p(d0), ctypes.c_void_p(d1), N, Hf, Wf, f3(dscale),
+                                        ctypes.c_void_p(dt), rt.stream), "make_target")
+    sh = np.array(list(shift) + [0.0], np.float32)
+    sc = np.array(list(scale) + [1.0], np.float32)
+    ds = np.array(list(dscale) + [1.0], np.float32)
+    x_ref = (s0 - sh) * (np.float32(1.0) / sc)
+    x_ref[..., 3] = 0
+    x_dev = _download(rt, net.xin, (N, net.H, net.W, 64), H16)
+    assert np.array_equal(x_dev, osur.unshuffle(x_ref).astype(H16))
+    t_ref = (s1 - s0) * (np.float32(1.0) / ds)
+    t_ref[..., 3] = 0
+    t_dev = _download(rt, dt, (N, net.H, net.W, 64), np.float32)
+    assert np.allclose(t_dev, osur.unshuffle(t_ref), rtol=1e-6, atol=0)
+    # fused step: state += y * dscale and the next input, y from the device's own layer 3 input
+    net.hidden_layers(N)
+    a3 = _download(rt, net.act[3], (N, net.H, net.W, 64), H16).astype(np.float64)
+    y_ref = osur.conv3x3(a3, _round16(params)[3][0], params[3][1].astype(np.float64))
+    net.apply(d0, (f3(dscale), f3(shift), f3(scale)), n=N)
+    st = _download(rt, d0, s0.shape, np.float32).astype(np.float64)
+    st_ref = s0 + osur.shuffle(y_ref) * np.array(list(dscale) + [0.0])
+    assert np.allclose(st, st_ref, rtol=0, atol=1e-6 * np.max(np.abs(osur.shuffle(y_ref))) * 2e-3 + 1e-7)
+    assert np.all(st[..., 3] == 0.0)
+    xn = _download(rt, net.xin, (N, net.H, net.W, 64), H16).astype(np.float64)
+    xn_ref = (st - sh) / sc
+    xn_ref[..., 3] = 0
+    assert np.allclose(xn, osur.unshuffle(xn_ref), rtol=2.0 ** -10, atol=1e-6)
+    for p in (d0, d1, dt):
+        rt.free(p)
+    net.free()
+
+
+@pytest.mark.parametrize("shape", [(2, 32, 128, 64), (1, 20, 256, 32)])
+def test_gradients_match_oracle(rt, shape):
+    """Loss-scaled device gradients / (48 npix) against the oracle's exact gradients."""
+    from oracle import surrogate as osur
+    N, Hf, Wf, C = shape
+    net, params = _net(rt, N, Hf, Wf, hidden=C, seed=11, last=1.0)
+    H, W = net.H, net.W
+    rng = np.random.default_rng(13)
+    x = _upload_input(rt, net, _coarse_input(rng, N, H, W))
+    tgt = rng.normal(0.0, 1.0, (N, H, W, 64)).astype(np.float32)
+    rt.h2d(net.tgt, tgt)
+    net.forward(N, keep=True)
+    loss = net.loss(N, gscale=1.0, sync=True)
+    net.backward(N)
+    got = net.grads()
+    ref_loss, ref = osur.loss_and_grads(_round16(params), x, tgt.astype(np.float64))
+    assert abs(loss - ref_loss) <= 5e-3 * ref_loss
+    count = N * H * W * 48
+    for li in range(4):
+        assert _rl2(got[li][0] / count, ref[li][0]) <= 2e-2, "layer %d weights" % li
+        assert _rl2(got[li][1] / count, ref[li][1]) <= 2e-2, "layer %d bias" % li
+    net.free()
+
+
+def test_wgrad_deterministic(rt):
+    N, Hf, Wf = 1, 24, 256
+    net, _ = _net(rt, N, Hf, Wf, seed=2, last=1.0)
+    rng = np.random.default_rng(4)
+    _upload_input(rt, net, _coarse_input(rng, N, net.H, net.W))
+    rt.h2d(net.tgt, rng.normal(0.0, 1.0, (N, net.H, net.W, 64)).astype(np.float32))
+    outs = []
+    for _ in range(2):
+        net.forward(N, keep=True)
+        net.loss(N, sync=True)
+        net.backward(N)
+        outs.append(net.grads())
+    for a, b in zip(*outs):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    net.free()
+
+
+def test_adam_matches_oracle(rt):
+    """Device Adam steps (loss-scaled gradients, eps scaled alike) = the oracle's steps."""
+    from oracle import surrogate as osur
+    N, Hf, Wf = 1, 32, 128
+    net, params = _net(rt, N, Hf, Wf, seed=21, last=1.0)
+    rng = np.random.default_rng(22)
+    _upload_input(rt, net, _coarse_input(rng, N, net.H, net.W))
+    rt.h2d(net.tgt, rng.normal(0.0, 1.0, (N, net.H, net.W, 64)).astype(np.float32))
+    count = N * net.H * net.W * 48
+    ref_params = [[w.astype(np.float64), b.astype(np.float64)] for w, b in params]
+    opt = osur.Adam(ref_params, lr=1e-2)
+    for _ in range(3):
+        net.forward(N, keep=True)
+        net.loss(N, sync=True)
+        net.backward(N)
+        g = [[a / count, b / count] for a, b in net.grads()]
+        net.adam(lr=1e-2, eps_scale=float(count))
+        opt.step(ref_params, g)
+        got = net.get_params()
+        for li in range(4):
+            assert _rl2(got[li][0], ref_params[li][0]) <= 1e-5
+            assert _rl2(got[li][1], ref_params[li][1]) <= 1e-4
+    net.free()
+
+
+def test_training_reduces_loss(rt):
+    """A fixed batch is fitted by the device trainer (forward, backward, Adam)."""
+    from oracle import surrogate as osur
+    N, Hf, Wf = 2, 64, 128
+    net, _ = _net(rt, N, Hf, Wf, seed=31)
+    rng = np.random.default_rng(32)
+    fine = rng.normal(0.0, 1.0, (N, Hf, Wf, 4))
+    fine[..., 3] = 0.0
+    x = _upload_input(rt, net, osur.unshuffle(fine).astype(H16).astype(np.float64))
+    # a smooth local target: a discrete Laplacian of the fine fields
+    lap = (np.roll(fine, 1, 1) + np.roll(fine, -1, 1) + np.roll(fine, 1, 2) + np.roll(fine, -1, 2) - 4 * fine)
+    rt.h2d(net.tgt, osur.unshuffle(0.5 * lap).astype(np.float32))
+    first = None
+    for i in range(300):
+        loss = net.train_step(N, lr=3e-3, sync=(i == 0 or i == 299))
+        if i == 0:
+            first = loss
+    assert x.shape[-1] == 64
+    assert loss < 0.2 * first, (first, loss)
+    net.free()
+
+
+# ------------------------------------------------------------------ coupling
+@pytest.fixture(scope="module")
+def lattice():
+    from pylbm_amd.surrogate import LatticeState
+    return LatticeState(n=128, dtype="float32")
+
+
+def test_restart_and_read_state(rt, lattice):
+    """restart (moments -> equilibrium -> m2f) then read_state returns the state,
+    and the populations equal the Simulation's own moment initialisation."""
+    from pylbm_amd import Simulation, cases
+    lat = lattice
+    s = lat.initial_state(9, kmin=1.0, kmax=4.0)
+    d = rt.malloc(s.nbytes)
+    rt.h2d(d, s)
+    lat.restart(d)
+    back = np.empty_like(s)
+    d2 = rt.malloc(s.nbytes)
+    lat.read_state(d2)
+    rt.d2h(back, d2)
+    rt.sync()
+    assert np.max(np.abs(back[..., 0] - 1.0)) <= 1e-6
+    assert np.max(np.abs(back[..., 1:3] - s[..., 1:3])) <= 1e-7
+    assert np.all(back[..., 3] == 0)
+    # the same fields through Simulation's init dict (FP64 host moments)
+    ux, uy = s[..., 1].astype(np.float64), s[..., 2].astype(np.float64)
+    dico = cases.decaying_d2q9(128)
+    dico["init"] = {cases.RHO: 1.0, cases.QX: lambda x, y: np.pad(ux, 1, mode="wrap"),
+                    cases.QY: lambda x, y: np.pad(uy, 1, mode="wrap")}
+    ref = Simulation(dico, dtype="float32")
+    for k in range(9):
+        assert np.array_equal(lat.sim.F[k], ref.F[k]), k
+    rt.free(d)
+    rt.free(d2)
+
+
+def test_rel_l2_kernel(rt, lattice):
+    from pylbm_amd.surrogate import Normalization, SurrogateNet, SurrogateRun
+    lat = lattice
+    net = SurrogateNet(rt, lat.H, lat.W, batch=1, train=False)
+    run = SurrogateRun(net, lat, Normalization((1, 0, 0), (1, 1, 1), (1, 1, 1)))
+    rng = np.random.default_rng(1)
+    a = rng.normal(0, 1, (lat.H, lat.W, 4)).astype(np.float32)
+    b = rng.normal(0, 1, (lat.H, lat.W, 4)).astype(np.float32)
+    rt.h2d(run.state, a)
+    rt.h2d(run.pred, b)
+    got = run.rel_l2(run.state, run.pred)
+    ref = np.sqrt(np.sum((a[..., 1:3].astype(np.float64) - b[..., 1:3]) ** 2) / np.sum(b[..., 1:3].astype(np.float64) ** 2))
+    assert abs(got - ref) <= 1e-12 * ref
+    run.free()
+    net.free()
+
+
+def test_surrogate_gain_over_plain_lattice(rt, lattice):
+    """Train on HIP lattice trajectories (128^2, k in [1, 4]: the cells per
+    wavelength of the 512^2 case with k in [4, 16]), then advance a held-out
+    initial state by 512 steps: the judged surrogate stays within the 1e-2
+    velocity rel-L2 of the plain run while running the lattice for 1/8 of the
+    steps, and is far closer than keeping the state frozen (persistence)."""
+    from pylbm_amd.surrogate import SurrogateRun, train_surrogate
+    lat = lattice
+    K, nint = 16, 32
+    net, norm, info = train_surrogate(lat, ntraj=24, steps=512, K=K, pool=8, batch=8, steps_per_traj=40,
+                                      lr=2e-3, lr_min=1e-5, kband=(1.0, 4.0))
+    assert info["final_loss"] < 0.05
+    s0 = lat.initial_state(777, kmin=1.0, kmax=4.0)
+    ref_state = rt.malloc(s0.nbytes)
+    init = rt.malloc(s0.nbytes)
+    rt.h2d(init, s0)
+    lat.restart(init)
+    lat.run(K * nint)
+    lat.read_state(ref_state)
+    run = SurrogateRun(net, lat, norm, K=K, judge_every=8, tol=1e-2)
+    rt.h2d(run.state, s0)
+    lat.restart(run.state)
+    run.start_from_lattice()
+    run.advance(nint)
+    err = run.rel_l2(run.state, ref_state)
+    persistence = run.rel_l2(init, ref_state)
+    print("surrogate rel-L2 %.3e, persistence %.3e, judges %s, fallback %s"
+          % (err, persistence, ["%.2e" % j for j in run.judges], run.fallback_at))
+    assert run.fallback_at is None
+    assert err <= 1e-2
+    assert err <= 0.1 * persistence
+    for p in (ref_state, init):
+        rt.free(p)
+    run.free()
+    net.free()

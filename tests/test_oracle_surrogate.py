@@ -1,5 +1,6 @@
 """Pins of the surrogate-network oracle (oracle/surrogate.py): finite
-differences of the loss, SiLU', and a direct loop convolution."""
+differences of the loss, SiLU', a direct loop convolution, the patch
+unshuffle/shuffle against explicit indexing, and the loss mask."""
 import numpy as np
 import pytest
 
@@ -28,8 +29,43 @@ def test_silu_grad():
     assert np.allclose(S.silu_grad(z), fd, rtol=0, atol=1e-8)
 
 
-def _small_params(rng):
-    p = S.init_params(seed=3, scale_last=1.0)
+def test_unshuffle_indexing_and_round_trip():
+    rng = np.random.default_rng(5)
+    fine = rng.normal(size=(2, 8, 12, 4))
+    c = S.unshuffle(fine)
+    assert c.shape == (2, 2, 3, 64)
+    for n, Y, X, py, px, ch in [(0, 0, 0, 0, 0, 0), (1, 1, 2, 3, 1, 2), (0, 1, 0, 2, 3, 3)]:
+        assert c[n, Y, X, (4 * py + px) * 4 + ch] == fine[n, 4 * Y + py, 4 * X + px, ch]
+    assert np.array_equal(S.shuffle(c), fine)
+
+
+def test_loss_ignores_the_empty_channel():
+    rng = np.random.default_rng(6)
+    params = _small_params(rng)
+    x = rng.normal(size=(1, 3, 4, 64))
+    tgt = rng.normal(size=(1, 3, 4, 64))
+    l0 = S.loss_and_grads(params, x, tgt)[0]
+    tgt2 = tgt.copy()
+    tgt2[..., 3::4] += 5.0
+    assert S.loss_and_grads(params, x, tgt2)[0] == l0
+    y = S.forward(params, x)[0]
+    m = S.loss_mask()
+    assert np.isclose(l0, np.sum(((y - tgt) * m) ** 2) / (3 * 4 * 48))
+
+
+def test_step_state_identity_with_zero_last_layer():
+    rng = np.random.default_rng(7)
+    params = _small_params(rng)
+    params[3][0][...] = 0.0
+    params[3][1][...] = 0.0
+    fine = rng.normal(size=(1, 8, 8, 4))
+    fine[..., 3] = 0.0
+    out = S.step_state(params, fine, (1.0, 0.0, 0.0), (0.1, 0.2, 0.2), (0.01, 0.02, 0.02))
+    assert np.array_equal(out, fine)
+
+
+def _small_params(rng, hidden=8):
+    p = S.init_params(seed=3, hidden=hidden, scale_last=1.0)
     return [[w * 0.5, rng.normal(scale=0.1, size=b.shape)] for w, b in p]
 
 
@@ -37,8 +73,8 @@ def _small_params(rng):
 def test_gradients_finite_difference(which):
     rng = np.random.default_rng(1)
     params = _small_params(rng)
-    x = rng.normal(size=(2, 4, 5, 4))
-    tgt = rng.normal(size=(2, 4, 5, 3))
+    x = rng.normal(size=(2, 4, 5, 64))
+    tgt = rng.normal(size=(2, 4, 5, 64))
     loss, grads = S.loss_and_grads(params, x, tgt)
     h = 1e-6
     for _ in range(6):
@@ -51,7 +87,8 @@ def test_gradients_finite_difference(which):
             fd = (S.loss_and_grads(params, xp, tgt)[0] - S.loss_and_grads(params, xm, tgt)[0]) / (2 * h)
             # analytic: back through layer 0 as well
             y, (acts, pre) = S.forward(params, x)
-            g = 2 * (y - tgt) / y.size
+            m = S.loss_mask()
+            g = 2 * (y - tgt) * m / (y[..., 0].size * m.sum())
             for li in range(3, -1, -1):
                 if li < 3:
                     g = g * S.silu_grad(pre[li])
@@ -75,8 +112,8 @@ def test_gradients_finite_difference(which):
 def test_adam_decreases_loss():
     rng = np.random.default_rng(2)
     params = _small_params(rng)
-    x = rng.normal(size=(1, 6, 6, 4))
-    tgt = S.forward(S.init_params(seed=9), x)[0]
+    x = rng.normal(size=(1, 6, 6, 64))
+    tgt = S.forward(S.init_params(seed=9, hidden=8), x)[0]
     opt = S.Adam(params, lr=3e-3)
     l0 = S.loss_and_grads(params, x, tgt)[0]
     for _ in range(60):
