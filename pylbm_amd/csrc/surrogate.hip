@@ -41,6 +41,14 @@ constexpr int SUR_WGRAD_BLOCKS = 256;  // weight-gradient workgroups per tap
 __device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ int wrap(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
+// Block-wide copy of TOTAL 16-byte items (item e = thread + 256 i) from ld to st;
+// the loop is unrolled so that the loads of several items are in flight together.
+template <int TOTAL, typename Ld, typename St>
+__device__ __forceinline__ void staged_copy(Ld ld, St st) {
+#pragma unroll 8
+  for (int e = threadIdx.x; e < TOTAL; e += 256) st(e, ld(e));
+}
+
 struct ApplyArgs {
   float4v dscale, shift, inv_scale;
   float* state;  // fine FP32 [N][4H][4W][4]
@@ -48,80 +56,86 @@ struct ApplyArgs {
 };
 
 // ---------------------------------------------------------------- conv 3x3
-// Workgroup = 4 waves = 4 consecutive tiles of 16 pixels (one row segment each) for
-// one group of MT * 16 output channels.  Each lane first loads its B operands of all
-// 9 taps (global, all in flight at once), the workgroup stages its weight rows in
-// LDS (rows padded by 16 bytes: the 16 rows one MFMA reads fall in distinct banks),
-// then the 9 * CINP / 32 MFMA steps run from registers and LDS.
-template <int CINP, int COUTP, int MT, int EPI>
+// Workgroup = 4 waves = one row segment of 64 pixels (4 tiles of 16, one per wave)
+// for all COUTP output channels.  The workgroup stages in LDS its input rows
+// y - 1 .. y + 1 over pixels x0 - 1 .. x0 + 64 (periodic wrap) and all weight rows;
+// both are padded by 16 bytes per pixel / row so that the 16 rows (pixels) one
+// MFMA operand read touches fall in distinct banks.  Then each wave runs its
+// 9 * CINP / 32 K steps of MT = COUTP / 16 independent MFMA chains from LDS.
+template <int CINP, int COUTP, int EPI>
 __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in, const h16* __restrict__ wpk,
                                                       const float* __restrict__ bias,
                                                       const h16* __restrict__ zin, void* __restrict__ out,
                                                       h16* __restrict__ zout, int N, int H, int W,
                                                       ApplyArgs ap) {
+  constexpr int MT = COUTP / 16;
   constexpr int KP = 9 * CINP;
   constexpr int KPP = KP + 8;
   constexpr int KC = CINP / 32;
-  constexpr int CG = COUTP / (16 * MT);
-  static_assert(CINP % 32 == 0 && COUTP % (16 * MT) == 0, "channel tiling");
-  __shared__ __attribute__((aligned(16))) h16 sw[MT * 16][KPP];
+  constexpr int CIP = CINP + 8;
+  constexpr int SEG = 64, SPAN = SEG + 2;
+  static_assert(CINP % 32 == 0 && COUTP % 16 == 0, "channel tiling");
+  extern __shared__ __attribute__((aligned(16))) h16 smem[];
+  h16* sw = smem;                       // [COUTP][KPP]
+  h16* sx = smem + COUTP * KPP;         // [3][SPAN][CIP]
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int segs = W / 16;
-  const int ntiles = N * H * segs;
-  const int cg = blockIdx.x % CG;
-  const int tile = (blockIdx.x / CG) * 4 + wv;
-  const bool active = tile < ntiles;
-  const int tt = active ? tile : ntiles - 1;  // inactive waves load a valid tile, store nothing
-  const int n = tt / (H * segs);
-  const int y = (tt / segs) % H;
-  const int x0 = (tt % segs) * 16;
+  const int segs = (W + SEG - 1) / SEG;
+  const int n = blockIdx.x / (H * segs);
+  const int y = (blockIdx.x / segs) % H;
+  const int xs = (blockIdx.x % segs) * SEG;
   const int g = lane >> 4, r = lane & 15;
-  const h16* img = in + (int64_t)n * H * W * CINP + g * 8;
-  half8 b[9][KC];
-#pragma unroll
-  for (int t = 0; t < 9; ++t) {
-    const int yy = wrap(y + t / 3 - 1, H);
-    const int xx = wrap(x0 + r + t % 3 - 1, W);
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) __builtin_memcpy(&b[t][kc], img + ((int64_t)yy * W + xx) * CINP + kc * 32, 16);
-  }
-  // weight rows: all loads of a thread in flight before its LDS stores
-  constexpr int NW8 = MT * 16 * (KP / 8);
-  constexpr int NFILL = (NW8 + 255) / 256;
-  const h16* wsrc = wpk + (int64_t)cg * MT * 16 * KP;
-  half8 wv8[NFILL];
-#pragma unroll
-  for (int i = 0; i < NFILL; ++i) {
-    const int e = threadIdx.x + i * 256;
-    if (e < NW8) __builtin_memcpy(&wv8[i], wsrc + (e / (KP / 8)) * KP + (e % (KP / 8)) * 8, 16);
-  }
-#pragma unroll
-  for (int i = 0; i < NFILL; ++i) {
-    const int e = threadIdx.x + i * 256;
-    if (e < NW8) __builtin_memcpy(&sw[e / (KP / 8)][(e % (KP / 8)) * 8], &wv8[i], 16);
+  // stage weights and input rows
+  {
+    constexpr int NW8 = COUTP * (KP / 8);
+    constexpr int NX8 = 3 * SPAN * (CINP / 8);
+    const h16* img = in + (int64_t)n * H * W * CINP;
+    staged_copy<NW8>(
+        [&](int e) {
+          half8 v;
+          __builtin_memcpy(&v, wpk + (e / (KP / 8)) * KP + (e % (KP / 8)) * 8, 16);
+          return v;
+        },
+        [&](int e, const half8& v) { __builtin_memcpy(sw + (e / (KP / 8)) * KPP + (e % (KP / 8)) * 8, &v, 16); });
+    staged_copy<NX8>(
+        [&](int f) {
+          const int c8 = (f % (CINP / 8)) * 8, q = (f / (CINP / 8)) % SPAN, dyi = f / ((CINP / 8) * SPAN);
+          // rows with W < 64 wrap more than once: full modulo
+          const int yy = wrap(y + dyi - 1, H), xx = (xs + q - 1 + W) % W;
+          half8 v;
+          __builtin_memcpy(&v, img + ((int64_t)yy * W + xx) * CINP + c8, 16);
+          return v;
+        },
+        [&](int f, const half8& v) {
+          const int c8 = (f % (CINP / 8)) * 8, q = (f / (CINP / 8)) % SPAN, dyi = f / ((CINP / 8) * SPAN);
+          __builtin_memcpy(sx + (dyi * SPAN + q) * CIP + c8, &v, 16);
+        });
   }
   __syncthreads();
-  if (!active) return;
+  const int x0 = xs + wv * 16;
+  if (x0 >= W) return;
   float4v acc[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) acc[mt] = float4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
+    const h16* bx = sx + ((t / 3) * SPAN + wv * 16 + r + t % 3) * CIP + g * 8;
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
+      half8 b;
+      __builtin_memcpy(&b, bx + kc * 32, 16);
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         half8 a;
-        __builtin_memcpy(&a, &sw[mt * 16 + r][t * CINP + kc * 32 + g * 8], 16);
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b[t][kc], acc[mt], 0, 0, 0);
+        __builtin_memcpy(&a, sw + (mt * 16 + r) * KPP + t * CINP + kc * 32 + g * 8, 16);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[mt], 0, 0, 0);
       }
     }
   }
-  // lane holds channels c0 + [0, 4) of pixel p, c0 = (cg * MT + mt) * 16 + g * 4
+  // lane holds channels c0 + [0, 4) of pixel p, c0 = mt * 16 + g * 4
   const int64_t p = ((int64_t)n * H + y) * W + x0 + r;
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
-    const int c0 = (cg * MT + mt) * 16 + g * 4;
+    const int c0 = mt * 16 + g * 4;
     if constexpr (EPI == EPI_LIN) {
       float4v v = acc[mt];
 #pragma unroll
@@ -448,15 +462,19 @@ __global__ void __launch_bounds__(256) rel_l2_kernel(const float* __restrict__ a
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-// MT = output-channel tiles of 16 per wave: two co groups for 64 output channels
-// give twice the waves on the small coarse grids
 template <int CINP, int COUTP, int EPI>
 int conv_launch(const h16* in, const h16* wpk, const float* bias, const h16* zin, void* out, h16* zout, int N,
                 int H, int W, const ApplyArgs& ap, hipStream_t s) {
-  constexpr int MT = 2;
-  const int64_t groups = ((int64_t)N * H * (W / 16) + 3) / 4;
-  conv3x3_kernel<CINP, COUTP, MT, EPI><<<(unsigned)(groups * (COUTP / (16 * MT))), 256, 0, s>>>(
-      in, wpk, bias, zin, out, zout, N, H, W, ap);
+  constexpr size_t lds = ((size_t)COUTP * (9 * CINP + 8) + 3 * 66 * (CINP + 8)) * sizeof(h16);
+  static_assert(lds <= 160 * 1024, "LDS budget");
+  static bool attr = false;  // allow more than 64 KiB of dynamic LDS once per instantiation
+  if (!attr) {
+    LBM_CHECK_HIP(hipFuncSetAttribute((const void*)conv3x3_kernel<CINP, COUTP, EPI>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
+  }
+  const int64_t groups = (int64_t)N * H * ((W + 63) / 64);
+  conv3x3_kernel<CINP, COUTP, EPI><<<(unsigned)groups, 256, lds, s>>>(in, wpk, bias, zin, out, zout, N, H, W, ap);
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -24,6 +24,7 @@ the zero pattern of s (which rows are relaxed at all, pylbm drops identity
 rows, printing/cython.py:347-349) is baked in.
 """
 import hashlib
+import os
 from fractions import Fraction
 
 import sympy as sp
@@ -34,8 +35,23 @@ from ..scheme import KernelSpec
 __all__ = ["generate_source", "spec_hash"]
 
 
+def _source_tag() -> str:
+    """Digest of the code generator and the device header it includes: a module
+    built from other sources never matches the cache key."""
+    h = hashlib.sha1()
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in (os.path.abspath(__file__), os.path.join(here, "..", "csrc", "lbm_device.h"),
+                 os.path.join(here, "..", "..", "include", "pylbm_hip.h")):
+        with open(path, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:12]
+
+
+_SOURCE_TAG = _source_tag()
+
+
 def spec_hash(spec: KernelSpec) -> str:
-    return hashlib.sha1(spec.key().encode()).hexdigest()[:16]
+    return hashlib.sha1((spec.key() + _SOURCE_TAG).encode()).hexdigest()[:16]
 
 
 def _lit(x) -> str:
@@ -515,7 +531,10 @@ static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_p
   int rc = make_args<C>(L, p, &a);
   if (rc) return rc;
   constexpr int V = vec_cells<S>();
-  const int vec = (a.nin[2] %% V == 0) ? V : 1;
+  // V cells per thread only where that still leaves >= 2^20 threads: small grids
+  // (a 512^2 plane) need the waves of one cell per thread to hide memory latency
+  const int64_t cells = (int64_t)a.nin[0] * a.nin[1] * a.nin[2];
+  const int vec = (a.nin[2] %% V == 0 && cells / V >= ((int64_t)1 << 20)) ? V : 1;
   const int threads = a.nin[2] / vec;
   unsigned bx = LBM_BX;
   while (bx > 64 && (int)(bx / 2) >= threads) bx /= 2;

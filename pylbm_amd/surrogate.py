@@ -311,7 +311,7 @@ class LatticeState:
                       "lbm_sur_state_to_moments")
         sim._equilibrium(m)
         sim._m2f(m, sim.container.F)
-        sim.container.Fnew.copy_from(sim.container.F)
+        # the fold step writes every cell of Fnew and reads no halo of F
         sim._update_m = True
 
     def run(self, nsteps):
@@ -456,15 +456,16 @@ class SurrogateRun:
         self.rt, self.lib = lattice.rt, lattice.lib
         self.state = self.rt.malloc(lattice.npix * 16)
         self.pred = self.rt.malloc(lattice.npix * 16)
+        self.good = self.rt.malloc(lattice.npix * 16)
         self.lpart = self.rt.malloc(512 * 8)
         self._shift, self._scale, self._dscale = _f3(norm.shift), _f3(norm.scale), _f3(norm.dscale)
         self.judges = []
         self.fallback_at = None
 
     def free(self):
-        for p in (self.state, self.pred, self.lpart):
+        for p in (self.state, self.pred, self.good, self.lpart):
             self.rt.free(p)
-        self.state = self.pred = self.lpart = None
+        self.state = self.pred = self.good = self.lpart = None
 
     def _make_input(self, d_state):
         self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(d_state), 1, self.lat.H, self.lat.W, self._shift,
@@ -488,39 +489,50 @@ class SurrogateRun:
         self.fallback_at = None
 
     def advance(self, nintervals):
-        """nintervals intervals of K lattice steps; state ends in self.state (and in the lattice)."""
+        """nintervals intervals of K lattice steps; state ends in self.state (and in the lattice).
+
+        Every judge_every-th interval is computed by the plain lattice run from the
+        surrogate state; its result is the new checkpoint.  When the drift of that
+        interval exceeds tol (or is not finite), the run falls back: it returns to
+        the previous checkpoint and runs the plain lattice Boltzmann method from
+        there to the end."""
         lat, K, je = self.lat, self.K, self.judge_every
         norm_args = (self._dscale, self._shift, self._scale)
+        nbytes = lat.npix * 16
+        self.rt.d2d(self.good, self.state, nbytes)
+        good_at = 0
         self._make_input(self.state)
         i = 0
         while i < nintervals:
-            if self.fallback_at is not None:
-                lat.run(K * (nintervals - i))
-                lat.read_state(self.state)
-                return
             nsur = min(je - 1 - (i % je), nintervals - i)
             if nsur > 0:
                 self.net.run(self.state, norm_args, nsur)
                 i += nsur
                 continue
             # the plain run computes this interval from the surrogate state
-            self.rt.d2d(self.pred, self.state, lat.npix * 16)
+            self.rt.d2d(self.pred, self.state, nbytes)
             self.net.run(self.pred, norm_args, 1)
             lat.restart(self.state)
             lat.run(K)
             lat.read_state(self.state)
             drift = self.rel_l2(self.pred, self.state)
             self.judges.append(drift)
-            if not drift <= self.tol:  # NaN drift falls back too
-                self.fallback_at = i
-            else:
-                self._make_input(self.state)
             i += 1
+            if not drift <= self.tol:  # NaN drift falls back too
+                self.fallback_at = i - 1
+                self.rt.d2d(self.state, self.good, nbytes)
+                lat.restart(self.state)
+                lat.run(K * (nintervals - good_at))
+                lat.read_state(self.state)
+                return
+            self.rt.d2d(self.good, self.state, nbytes)
+            good_at = i
+            self._make_input(self.state)
         lat.restart(self.state)
 
 
-def train_surrogate(lattice, ntraj=200, steps=2000, K=16, pool=16, batch=8, steps_per_traj=25, lr=1e-3,
-                    lr_min=1e-5, seed=0, kband=(4.0, 16.0), hidden=64, pushforward=0.0, noise=0.0, log=None):
+def train_surrogate(lattice, ntraj=200, steps=2000, K=16, pool=8, batch=8, steps_per_traj=100, lr=2e-3,
+                    lr_min=1e-5, seed=0, kband=(4.0, 16.0), hidden=32, pushforward=0.0, noise=0.002, log=None):
     """Generate ntraj trajectories with the HIP lattice Boltzmann run and train a network on them.
 
     Returns (net, norm, info). Trajectory seeds are seed * 100000 + j."""
