@@ -119,6 +119,19 @@ int lbm_sur_make_input(const float* d_state, int N, int Hf, int Wf, const float*
 int lbm_sur_make_target(const float* d_s0, const float* d_s1, int N, int Hf, int Wf, const float* dscale,
                         float* d_t, void* stream);
 
+/* the same distance without draining the stream: d_partial needs 514 doubles; the
+ * sums (sum |u_a - u_b|^2, sum |u_b|^2) land in h_out[0..1] (pinned host memory, see
+ * lbm_host_malloc) once the stream reaches this point (lbm_event_record / lbm_event_sync) */
+int lbm_sur_rel_l2_async(const float* d_a, const float* d_b, int64_t npix, double* d_partial, double* h_out,
+                         void* stream);
+
+/* state (rho, qx / rho, qy / rho) straight from plain FP32 populations of a 2D layout:
+ * m_i = sum_k mc[i * ns + k] f_k, mc = the rows of M of rho, qx and qy (3 * ns host
+ * doubles); the moments Simulation.m reads (pylbm/simulation.py:215-224), conserved
+ * rows only */
+int lbm_sur_populations_to_state(const float* d_f, const lbm_layout* layout, const double* mc, int H, int W,
+                                 float* d_state, void* stream);
+
 /* training noise: state += sigma[f] * N(0, 1) on the fields f = rho, ux, uy of every
  * fine pixel, from a counter-based generator keyed by seed (reproducible) */
 int lbm_sur_perturb(float* d_state, int64_t npix, const float* sigma, uint64_t seed, void* stream);

@@ -409,6 +409,23 @@ int lbm_event_elapsed_ms(void* start, void* stop, float* ms) {
   return 0;
 }
 
+int lbm_event_sync(void* event) {
+  LBM_CHECK_HIP(hipEventSynchronize((hipEvent_t)event));
+  return 0;
+}
+
+int lbm_host_malloc(void** h_ptr, uint64_t bytes) {
+  LBM_REQUIRE(h_ptr, "h_ptr is NULL");
+  *h_ptr = nullptr;
+  LBM_CHECK_HIP(hipHostMalloc(h_ptr, bytes ? bytes : 64, hipHostMallocDefault));
+  return 0;
+}
+
+int lbm_host_free(void* h_ptr) {
+  if (h_ptr) LBM_CHECK_HIP(hipHostFree(h_ptr));
+  return 0;
+}
+
 static int ghost_fill_impl(void* d_f, const lbm_layout* L, const lbm_velocities* V, int all_pops,
                            void* stream) {
   int rc = check_layout(L);

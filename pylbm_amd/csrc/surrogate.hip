@@ -460,6 +460,49 @@ __global__ void __launch_bounds__(256) rel_l2_kernel(const float* __restrict__ a
   }
 }
 
+// the 256 partial pairs of rel_l2_kernel summed in a fixed order into out[0..1]
+__global__ void __launch_bounds__(256) rel_l2_final_kernel(const double* __restrict__ partial,
+                                                           double* __restrict__ out) {
+  __shared__ double s0[256], s1[256];
+  s0[threadIdx.x] = partial[2 * threadIdx.x];
+  s1[threadIdx.x] = partial[2 * threadIdx.x + 1];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      s0[threadIdx.x] += s0[threadIdx.x + off];
+      s1[threadIdx.x] += s1[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s0[0];
+    out[1] = s1[0];
+  }
+}
+
+// conserved moments of plain FP32 populations (2D layout) -> state (rho, qx / rho, qy / rho):
+// m_i = sum_k mc[i][k] f_k with the rows of M of rho, qx, qy
+struct ConservedRows {
+  double c[3][LBM_MAX_Q];
+};
+
+__global__ void __launch_bounds__(256) populations_to_state_kernel(const float* __restrict__ f, lbm_layout L,
+                                                                   ConservedRows mc, int H, int W,
+                                                                   float* __restrict__ st) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t)H * W) return;
+  const int yy = (int)(p / W), xx = (int)(p % W);
+  const int64_t c = ::lbm::cell_offset(L, 0, yy + L.halo[1], xx + L.halo[2]);
+  double m[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < L.ns; ++k) {
+    const double v = (double)f[k * L.pop_stride + c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) m[i] += mc.c[i][k] * v;
+  }
+  float4v s = {(float)m[0], (float)(m[1] / m[0]), (float)(m[2] / m[0]), 0.f};
+  __builtin_memcpy(st + p * 4, &s, 16);
+}
+
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 template <int CINP, int COUTP, int EPI>
@@ -641,6 +684,34 @@ int lbm_sur_make_target(const float* d_s0, const float* d_s1, int N, int Hf, int
   float4v is = {1.f / dscale[0], 1.f / dscale[1], 1.f / dscale[2], 0.f};
   make_target_kernel<<<blocks_for((int64_t)N * Hf * Wf), 256, 0, (hipStream_t)stream>>>(d_s0, d_s1, N, Hf, Wf, is,
                                                                                        d_t);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int lbm_sur_rel_l2_async(const float* d_a, const float* d_b, int64_t npix, double* d_partial, double* h_out,
+                         void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  rel_l2_kernel<<<256, 256, 0, s>>>(d_a, d_b, npix, d_partial);
+  LBM_CHECK_HIP(hipGetLastError());
+  rel_l2_final_kernel<<<1, 256, 0, s>>>(d_partial, d_partial + 512);
+  LBM_CHECK_HIP(hipGetLastError());
+  LBM_CHECK_HIP(hipMemcpyAsync(h_out, d_partial + 512, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+int lbm_sur_populations_to_state(const float* d_f, const lbm_layout* L, const double* mc, int H, int W,
+                                 float* d_state, void* stream) {
+  int rc = check_layout(L);
+  if (rc) return rc;
+  LBM_REQUIRE(L->storage == LBM_F32, "populations_to_state: plain FP32 populations only (storage %d)",
+              L->storage);
+  LBM_REQUIRE(L->dim == 2 && L->n[1] - 2 * L->halo[1] == H && L->n[2] - 2 * L->halo[2] == W,
+              "populations_to_state: layout is not a 2D %dx%d interior", H, W);
+  ConservedRows rows;
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < LBM_MAX_Q; ++k) rows.c[i][k] = k < L->ns ? mc[i * L->ns + k] : 0.0;
+  populations_to_state_kernel<<<blocks_for((int64_t)H * W), 256, 0, (hipStream_t)stream>>>(d_f, *L, rows, H, W,
+                                                                                          d_state);
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -84,6 +84,9 @@ CORE_SIGNATURES = {
     "lbm_event_destroy": ([_P], _I),
     "lbm_event_record": ([_P, _P], _I),
     "lbm_event_elapsed_ms": ([_P, _P, ctypes.POINTER(ctypes.c_float)], _I),
+    "lbm_event_sync": ([_P], _I),
+    "lbm_host_malloc": ([ctypes.POINTER(_P), _U64], _I),
+    "lbm_host_free": ([_P], _I),
     "lbm_ghost_fill": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmVelocities), _P], _I),
     "lbm_ghost_fill_all": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmVelocities), _P], _I),
     "lbm_links_apply": ([_P, ctypes.POINTER(LbmLayout), _P, _P, _P, _P, _D, _I64, _I, _P, _P], _I),
@@ -115,6 +118,8 @@ SURROGATE_SIGNATURES = {
     "lbm_sur_state_to_moments": ([_P, _I, _I, ctypes.POINTER(LbmLayout), _I, _I, _I, _P, _P], _I),
     "lbm_sur_rel_l2": ([_P, _P, _I64, _P, ctypes.POINTER(_D), _P], _I),
     "lbm_sur_perturb": ([_P, _I64, _FP, ctypes.c_uint64, _P], _I),
+    "lbm_sur_rel_l2_async": ([_P, _P, _I64, _P, _P, _P], _I),
+    "lbm_sur_populations_to_state": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(_D), _I, _I, _P, _P], _I),
 }
 
 GEN_SIGNATURES = {

@@ -290,11 +290,22 @@ class LatticeState:
         c = self.sim.scheme.consm
         self.irho, self.iqx, self.iqy = c[RHO], c[QX], c[QY]
         self.npix = self.H * self.W
+        M = self.sim.spec.M
+        rows = [[float(v) for v in M[i]] for i in (self.irho, self.iqx, self.iqy)]
+        self._mc = (ctypes.c_double * (3 * self.sim.ns))(*[v for r in rows for v in r])
+        self._direct = self.sim.storage == rtm.LBM_F32
 
     def read_state(self, d_state):
-        """Conserved moments of the current populations into an FP32 state buffer."""
+        """Conserved moments of the current populations into an FP32 state buffer
+        (plain FP32 populations: straight from F; other formats: f2m then the moments)."""
         sim = self.sim
-        sim._f2m(sim.container.F, sim.container.m)
+        F = sim.container.F
+        if self._direct:
+            self.rt.check(self.lib.lbm_sur_populations_to_state(ctypes.c_void_p(F.ptr), ctypes.byref(F.layout),
+                                                                self._mc, self.H, self.W, ctypes.c_void_p(d_state),
+                                                                self.rt.stream), "lbm_sur_populations_to_state")
+            return
+        sim._f2m(F, sim.container.m)
         sim._update_m = False
         m = sim.container.m
         self.rt.check(self.lib.lbm_sur_moments_to_state(ctypes.c_void_p(m.ptr), ctypes.byref(m.layout), self.irho,
@@ -456,16 +467,27 @@ class SurrogateRun:
         self.rt, self.lib = lattice.rt, lattice.lib
         self.state = self.rt.malloc(lattice.npix * 16)
         self.pred = self.rt.malloc(lattice.npix * 16)
-        self.good = self.rt.malloc(lattice.npix * 16)
-        self.lpart = self.rt.malloc(512 * 8)
+        self.ckpt = [self.rt.malloc(lattice.npix * 16) for _ in range(3)]
+        self.lpart = self.rt.malloc(514 * 8)
+        hp = ctypes.c_void_p()
+        self.rt.check(self.lib.lbm_host_malloc(ctypes.byref(hp), 2 * 2 * 8), "lbm_host_malloc")
+        self.hdrift = hp.value
+        self.events = [self.rt.event(), self.rt.event()]
         self._shift, self._scale, self._dscale = _f3(norm.shift), _f3(norm.scale), _f3(norm.dscale)
         self.judges = []
         self.fallback_at = None
 
     def free(self):
-        for p in (self.state, self.pred, self.good, self.lpart):
+        for p in [self.state, self.pred, self.lpart] + self.ckpt:
             self.rt.free(p)
-        self.state = self.pred = self.good = self.lpart = None
+        self.state = self.pred = self.lpart = None
+        self.ckpt = []
+        if self.hdrift:
+            self.rt.check(self.lib.lbm_host_free(ctypes.c_void_p(self.hdrift)), "lbm_host_free")
+            self.hdrift = None
+        for e in self.events:
+            self.lib.lbm_event_destroy(e)
+        self.events = []
 
     def _make_input(self, d_state):
         self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(d_state), 1, self.lat.H, self.lat.W, self._shift,
@@ -488,21 +510,53 @@ class SurrogateRun:
         self.judges = []
         self.fallback_at = None
 
+    def _drift_async(self, slot):
+        """Start the drift of pred against state into pinned slot `slot` (no host wait)."""
+        host = self.hdrift + 16 * slot
+        self.rt.check(self.lib.lbm_sur_rel_l2_async(ctypes.c_void_p(self.pred), ctypes.c_void_p(self.state),
+                                                    self.lat.npix, ctypes.c_void_p(self.lpart), ctypes.c_void_p(host),
+                                                    self.rt.stream), "lbm_sur_rel_l2_async")
+        self.rt.record(self.events[slot])
+
+    def _drift_wait(self, slot):
+        self.rt.check(self.lib.lbm_event_sync(self.events[slot]), "lbm_event_sync")
+        d, n = (ctypes.c_double * 2).from_address(self.hdrift + 16 * slot)
+        return 0.0 if d == 0.0 and n == 0.0 else float(np.sqrt(d / n)) if n > 0 else float("nan")
+
     def advance(self, nintervals):
         """nintervals intervals of K lattice steps; state ends in self.state (and in the lattice).
 
         Every judge_every-th interval is computed by the plain lattice run from the
-        surrogate state; its result is the new checkpoint.  When the drift of that
-        interval exceeds tol (or is not finite), the run falls back: it returns to
-        the previous checkpoint and runs the plain lattice Boltzmann method from
-        there to the end."""
+        surrogate state; its result is a checkpoint.  The drift of a judge (rel-L2
+        of the velocity between the network's prediction for that interval and the
+        lattice result) is read on the host only after the next cycle has been
+        queued, so the stream never drains.  When a drift exceeds tol (or is not
+        finite), the run falls back: it returns to the last checkpoint whose judge
+        passed (or the start) and runs the plain lattice Boltzmann method from
+        there to the end.  Judge j uses drift slot j % 2 and checkpoint j % 3; the
+        start is checkpoint 2 (judge -1)."""
         lat, K, je = self.lat, self.K, self.judge_every
         norm_args = (self._dscale, self._shift, self._scale)
         nbytes = lat.npix * 16
-        self.rt.d2d(self.good, self.state, nbytes)
-        good_at = 0
+        confirmed, confirmed_at = 2, 0          # checkpoint whose judge passed, its interval
+        self.rt.d2d(self.ckpt[confirmed], self.state, nbytes)
+        pending = None                          # (judge index, interval after it)
         self._make_input(self.state)
+        j = 0
         i = 0
+
+        def check(p):
+            drift = self._drift_wait(p[0] % 2)
+            self.judges.append(drift)
+            return drift <= self.tol  # False for NaN as well
+
+        def fall_back(at):
+            self.fallback_at = at
+            self.rt.d2d(self.state, self.ckpt[confirmed], nbytes)
+            lat.restart(self.state)
+            lat.run(K * (nintervals - confirmed_at))
+            lat.read_state(self.state)
+
         while i < nintervals:
             nsur = min(je - 1 - (i % je), nintervals - i)
             if nsur > 0:
@@ -515,19 +569,20 @@ class SurrogateRun:
             lat.restart(self.state)
             lat.run(K)
             lat.read_state(self.state)
-            drift = self.rel_l2(self.pred, self.state)
-            self.judges.append(drift)
+            self._drift_async(j % 2)
+            self.rt.d2d(self.ckpt[j % 3], self.state, nbytes)
             i += 1
-            if not drift <= self.tol:  # NaN drift falls back too
-                self.fallback_at = i - 1
-                self.rt.d2d(self.state, self.good, nbytes)
-                lat.restart(self.state)
-                lat.run(K * (nintervals - good_at))
-                lat.read_state(self.state)
-                return
-            self.rt.d2d(self.good, self.state, nbytes)
-            good_at = i
+            if pending is not None:
+                if not check(pending):
+                    fall_back(pending[1] - 1)
+                    return
+                confirmed, confirmed_at = pending[0] % 3, pending[1]
+            pending = (j, i)
+            j += 1
             self._make_input(self.state)
+        if pending is not None and not check(pending):
+            fall_back(pending[1] - 1)
+            return
         lat.restart(self.state)
 
 
