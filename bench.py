@@ -140,6 +140,22 @@ def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), 
     run = SurrogateRun(net, lat, norm, K=K, judge_every=judge_every, tol=tol)
     ref = rt.malloc(lat.npix * 16)
     work = rt.malloc(lat.npix * 16)
+    # one network step (3 hidden convolutions + the fused output/apply convolution):
+    # HIP events around 200 steps of the C loop on the lattice stream
+    ch = (64,) + (info["hidden"],) * 3 + (64,)
+    flops = 2 * 9 * net.H * net.W * sum(ch[i] * ch[i + 1] for i in range(4))
+    rt.h2d(work, lat.initial_state(eval_seeds[0]))
+    run._make_input(work)
+    net.run(work, (run._dscale, run._shift, run._scale), 10)
+    e0, e1 = rt.event(), rt.event()
+    rt.record(e0)
+    net.run(work, (run._dscale, run._shift, run._scale), 200)
+    rt.record(e1)
+    step_us = 1e3 * rt.elapsed_ms(e0, e1) / 200
+    rt.record(e0)
+    lat.run(16 * 25)
+    rt.record(e1)
+    lbm_step_us = 1e3 * rt.elapsed_ms(e0, e1) / 400
     by_h = {}
     for hz in horizons:
         nint = hz // K
@@ -195,7 +211,13 @@ def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), 
                     "E(k)~k^-5/3 on |k| in [4,16], rms 0.05",
         "network": "4x4-patch periodic 3x3 conv net 64-%d-%d-%d-64, SiLU, FP16 MFMA, FP32 accumulate"
                    % ((info["hidden"],) * 3),
-        "K": K, "judge_every": judge_every, "tol_rel_l2": tol, "horizons": by_h,
+        "K": K, "judge_every": judge_every, "tol_rel_l2": tol,
+        "network_step_us": round(step_us, 2), "lattice_step_us": round(lbm_step_us, 3),
+        "network_roofline": {"bound": "mfma", "achieved": round(flops / step_us * 1e-6, 1),
+                             "peak": 2500.0, "unit": "TFLOP/s",
+                             "frac": round(flops / step_us * 1e-6 / 2500.0, 4),
+                             "flops_per_step": flops, "dtype": "fp16 x fp16 -> fp32"},
+        "horizons": by_h,
         "train": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()},
     }
 

@@ -36,7 +36,7 @@ constexpr int EPI_BWD = 2;    // g = acc * silu'(z_in): store FP16
 constexpr int EPI_APPLY = 3;  // fine state += (acc + b) * dscale, next network input
 
 constexpr int WG_PIX = 32;             // pixels per weight-gradient block iteration
-constexpr int SUR_WGRAD_BLOCKS = 256;  // weight-gradient workgroups per tap
+constexpr int SUR_WGRAD_BLOCKS = 64;   // weight-gradient workgroups per tap
 
 __device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ int wrap(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
@@ -258,14 +258,20 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_kernel(const h16* __restric
   }
 }
 
-// out[i] = sum over nblk rows of partial[b * len + i], in row order (deterministic)
+// out[i] = sum over nblk rows of partial[b * len + i]: four interleaved running sums
+// (rows b = 4 j + q) added in a fixed order, so the result is deterministic
 __global__ void __launch_bounds__(256) sum_partials_kernel(const float* __restrict__ partial, int nblk, int len,
                                                            float* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
-  float s = 0.f;
-  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * len + i];
-  out[i] = s;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  int b = 0;
+  for (; b + 4 <= nblk; b += 4) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] += partial[(int64_t)(b + q) * len + i];
+  }
+  for (; b < nblk; ++b) s[0] += partial[(int64_t)b * len + i];
+  out[i] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 // Masked mean-square loss over the 48 value channels of 64; dY = 2 gscale (y - t) as FP16
