@@ -15,6 +15,8 @@ device sync.  Printed as one JSON line:
             counters on a plain copy kernel of known size, or null
   cpu_baseline  the oracle's C restatement of pylbm's generated Cython loop
             (FP64, AoS, -O3) on a bounded 128^3 sample, all host threads
+  configs   MLUPS of the C1 cavity (FP64), C2 channel (FP64, FP32) and C4 sphere
+            packing (FP32), each link list timed alone, C4's mass drift
   surrogate the learned surrogate on the 512^2 periodic D2Q9 decaying flow:
             trained on HIP lattice trajectories, then its simulated steps per
             second against the plain FP32 lattice run and its velocity rel-L2
@@ -124,6 +126,59 @@ def _cpu_baseline(target_s=8.0):
     return out, cores, n
 
 
+def _link_timing(sim, reps=20):
+    """Per-launch time and achieved bytes of each link list of sim, applied alone."""
+    rt = sim.rt
+    out = []
+    F = sim.container.F
+    es = F.layout.elem_size
+    for dl in sim._dev_links:
+        if dl.n == 0:
+            continue
+        dl.apply(F.ptr)
+        e0, e1 = rt.event(), rt.event()
+        rt.record(e0)
+        for _ in range(reps):
+            dl.apply(F.ptr)
+        rt.record(e1)
+        ms = rt.elapsed_ms(e0, e1) / reps
+        rhs_bytes = es if dl.d_rhs is not None else 0
+        per_link = 8 + 2 * es + rhs_bytes + (1 if dl.d_sign is not None else 0)
+        name = getattr(dl.method, "name", None) or str(getattr(dl.method, "kind", dl.method))
+        out.append({"method": str(name), "links": int(dl.n), "us_per_launch": round(1e3 * ms, 2),
+                    "Glinks_per_s": round(dl.n / (ms * 1e-3) / 1e9, 3),
+                    "GBps_alg": round(dl.n * per_link / (ms * 1e-3) / 1e9, 1), "bytes_per_link": per_link})
+    return out
+
+
+def _configs_leg():
+    """The other workloads: MLUPS of C1 (FP64), C2 (FP64 and FP32), C4 (FP32) through
+    Simulation.run, their link lists timed alone, and C4's mass drift."""
+    from pylbm_amd import Simulation, cases
+
+    res = {}
+    plan = [("C1 lid-driven cavity 128^2 D2Q9 MRT FP64", lambda: cases.lid_cavity_d2q9(128), "float64", 2000),
+            ("C2 Karman channel 1024x256 D2Q9 MRT FP64", lambda: cases.karman_d2q9(), "float64", 2000),
+            ("C2 Karman channel 1024x256 D2Q9 MRT FP32", lambda: cases.karman_d2q9(), "float32", 2000),
+            ("C4 sphere packing 512x256x256 D3Q19 MRT FP32", lambda: cases.sphere_packing_d3q19(), "float32", 200)]
+    for name, make, dtype, steps in plan:
+        t0 = time.perf_counter()
+        sim = Simulation(make(), dtype=dtype)
+        setup = time.perf_counter() - t0
+        cells = int(np.prod(sim.domain.shape_in))
+        sim.run(10)
+        mass0 = sim.conserved_sums()[0] if name.startswith("C4") else None
+        ms, _ = _time_run(sim, steps)
+        entry = {"MLUPS": round(cells * steps / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms / steps, 4),
+                 "steps": steps, "cells": cells, "setup_s": round(setup, 1), "links": _link_timing(sim)}
+        if mass0 is not None:
+            mass1 = sim.conserved_sums()[0]
+            entry["mass_rel_drift"] = float("%.3e" % ((mass1 - mass0) / mass0))
+        res[name] = entry
+        del sim
+    return res
+
+
 def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), K=16, judge_every=8,
                    tol=1e-2, horizons=(256, 2000)):
     """Train the surrogate on HIP lattice trajectories (512^2 periodic D2Q9, FP32, nu = 1e-3),
@@ -231,6 +286,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the C1/C2/C4 timings")
     ap.add_argument("--sur-traj", type=int, default=200, help="surrogate training trajectories")
     ap.add_argument("--sur-steps", type=int, default=2000, help="lattice steps per trajectory")
     ap.add_argument("--sur-train", type=int, default=100, help="optimizer steps per trajectory")
@@ -303,8 +359,10 @@ def main():
                          "one_time_step + periodic ghost fill (oracle/cgen.py), %d steps on %d OpenMP threads, "
                          "%d steps on 1 thread" % (cn, res[cores][1], cores, res[1][1])}
 
+    del sim
+    if not args.no_configs:
+        extra["configs"] = _configs_leg()
     if not args.no_surrogate:
-        del sim
         extra["surrogate"] = _surrogate_leg(args.sur_traj, args.sur_steps, args.sur_train)
 
     out = {

@@ -60,19 +60,24 @@ int lbm_sur_apply_step(const void* d_in, int cinp, const void* d_wpk, const floa
                        int W, void* stream);
 
 /* the buffers of one network for lbm_sur_run: act[0] = input (FP16, 64 channels),
- * act[1..3] = hidden activations (FP16, `hidden` channels), wpk / bias = forward
- * operands and biases of the 4 layers */
+ * act[1..3] = hidden activations (FP16, `hidden` channels; not used by
+ * lbm_sur_run), xalt = a second input buffer of act[0]'s size, wpk / bias =
+ * forward operands and biases of the 4 layers */
 typedef struct lbm_sur_net {
   void* act[4];
+  void* xalt;
   const void* wpk[4];
   const float* bias[4];
   int32_t hidden;
   int32_t _pad;
 } lbm_sur_net;
 
-/* nsteps inference steps without returning to the host: per step the three hidden
- * convolutions and lbm_sur_apply_step (which writes the next input into act[0]);
- * act[0] must hold the input of d_state on entry */
+/* nsteps inference steps without returning to the host, one launch per step: a
+ * workgroup computes all 4 layers of a 4 x 16 coarse-pixel tile (halo recomputed,
+ * activations in LDS), adds the increment to d_state and writes the next input.
+ * The input alternates between act[0] and xalt; on return act[0] holds the input
+ * of the new d_state.  act[0] must hold the input of d_state on entry.  The coarse
+ * grid must be a multiple of 4 x 16. */
 int lbm_sur_run(const lbm_sur_net* net, float* d_state, const float* dscale, const float* shift,
                 const float* scale, int64_t nsteps, int N, int H, int W, void* stream);
 

@@ -195,9 +195,17 @@ class DeviceLinks:
             sign = np.ones(self.n)
             rhs = np.zeros(self.n)
         self.conflict = (np.intersect1d(st, ld).size > 0) or (np.unique(st).size != st.size)
+        self.order = None
         if self.conflict:
             st, ld, sign, rhs = _resolve_sequential(st, ld, sign, rhs)
             self.n = st.size
+        elif self.n > 1:
+            # no slot is both stored and loaded and every store is unique, so the list
+            # may run in any order: sort by store address, which puts the links of one
+            # (o, m) row (all populations, interleaved layout) next to each other and
+            # lets the lanes of a wave share cache lines
+            self.order = np.argsort(st, kind="stable")
+            st, ld, sign, rhs = st[self.order], ld[self.order], sign[self.order], rhs[self.order]
         kstore = _population(st, layout)
         kload = _population(ld, layout)
         if layout.storage in DEVIATION_FORMATS:
@@ -230,6 +238,8 @@ class DeviceLinks:
             ld = _flat(self.method.iload[0].T if self.method._fixed else self.method.iload[0], self.layout)
             rhs = (rhs + self.method.kind.sign * wshift[_population(ld, self.layout)]
                    - wshift[_population(st, self.layout)])
+        if self.order is not None:
+            rhs = np.asarray(rhs)[self.order]
         ctype = np.float64 if self.layout.storage == LBM_F64 else np.float32
         if self.d_rhs is None:
             self.d_rhs = self._up(rhs.astype(ctype))

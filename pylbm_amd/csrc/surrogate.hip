@@ -182,6 +182,185 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in
   }
 }
 
+// ------------------------------------------------------- fused inference step
+// One launch per inference step: a workgroup computes the 4 layers for a core tile
+// of FT_Y x FT_X coarse pixels, recomputing the 3-pixel halo each hidden layer needs
+// (layer l produces a (FT_Y + 2 (3 - l)) x (FT_X + 2 (3 - l)) region).  Activations
+// stay in LDS; each layer's weights are staged in LDS after the previous layer.  The layer-0 input is read from `xin` and the next step's input is
+// written to `xout` (a different buffer: neighbouring workgroups still read `xin`).
+constexpr int FT_Y = 4, FT_X = 16;
+#ifndef FUSED_PAIR_MASK
+#define FUSED_PAIR_MASK 7  // bit l: layer l of the fused step computes tiles in pairs
+#endif
+
+template <int C>
+struct FusedGeom {
+  static constexpr int CIP0 = 64 + 8, CIPH = C + 8;
+  static constexpr int R_IN = FT_Y + 8, W_IN = FT_X + 8;
+  static constexpr int BUF_A = (R_IN * W_IN * CIP0 > (FT_Y + 4) * (FT_X + 4) * CIPH) ? R_IN * W_IN * CIP0
+                                                                                   : (FT_Y + 4) * (FT_X + 4) * CIPH;
+  static constexpr int BUF_B = (FT_Y + 6) * (FT_X + 6) * CIPH;
+  static constexpr int W0 = C * (9 * 64 + 8), WH = C * (9 * C + 8), W3 = 64 * (9 * C + 8);
+  static constexpr int WBUF = W0 > WH ? (W0 > W3 ? W0 : W3) : (WH > W3 ? WH : W3);
+  static constexpr size_t LDS = (size_t)(BUF_A + BUF_B + WBUF) * sizeof(h16);
+};
+
+template <int CIN, int COUT>
+struct LayerW {
+  static constexpr int KP = 9 * CIN, KPP = KP + 8;
+  static constexpr int N8 = COUT * (KP / 8);          // 16-byte items
+};
+
+// weights of one layer, global [COUT][9 CIN] -> LDS rows padded to 9 CIN + 8
+template <int CIN, int COUT>
+__device__ __forceinline__ void fused_copy_w(const h16* __restrict__ w, h16* __restrict__ sw) {
+  using L = LayerW<CIN, COUT>;
+  staged_copy<L::N8>(
+      [&](int e) {
+        half8 v;
+        __builtin_memcpy(&v, w + (e / (L::KP / 8)) * L::KP + (e % (L::KP / 8)) * 8, 16);
+        return v;
+      },
+      [&](int e, const half8& v) { __builtin_memcpy(sw + (e / (L::KP / 8)) * L::KPP + (e % (L::KP / 8)) * 8, &v, 16); });
+}
+
+// Layer LAYER of the tile: input region (RO + 2) x (WO + 2) at `src` (stride CIPI),
+// output region RO x WO at `dst` (stride CIPO), or the state / next input for LAYER 3.
+template <int C, int LAYER, int CIN, int COUT>
+__device__ __forceinline__ void fused_layer(const h16* __restrict__ src, h16* __restrict__ dst,
+                                            const h16* __restrict__ sw, const float* __restrict__ bias,
+                                            int n, int y0, int x0, int H, int W, const ApplyArgs& ap) {
+  constexpr int RO = FT_Y + 2 * (3 - LAYER), WO = FT_X + 2 * (3 - LAYER);
+  constexpr int WI = WO + 2;
+  constexpr int CIPI = CIN + 8, CIPO = COUT + 8;
+  constexpr int KPP = 9 * CIN + 8;
+  constexpr int NPIX = RO * WO;
+  constexpr int TILES = (NPIX + 15) / 16;
+  constexpr int MT = COUT / 16;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  // TP = 2: a wave takes a pair of tiles (2 j, 2 j + 1) per pass, pairs j = wv, wv + 4,
+  // ...: each weight read feeds two MFMAs and the MFMA pipe has 2 MT independent
+  // chains; TP = 1: one tile per pass (more waves busy on small layers)
+  constexpr int TP = ((FUSED_PAIR_MASK >> LAYER) & 1) ? 2 : 1;
+  constexpr int GROUPS = (TILES + TP - 1) / TP;
+  for (int j = wv; j < GROUPS; j += 4) {
+    int pcs[2], prs[2], pqs[2];
+    bool st[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int p = (TP * j + (TP == 2 ? u : 0)) * 16 + r;
+      st[u] = p < NPIX;
+      pcs[u] = st[u] ? p : NPIX - 1;  // padding lanes compute a valid pixel, store nothing
+      prs[u] = pcs[u] / WO;
+      pqs[u] = pcs[u] % WO;
+    }
+    float4v acc[2][MT];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) acc[u][mt] = float4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const h16* bx0 = src + ((prs[0] + t / 3) * WI + pqs[0] + t % 3) * CIPI + g * 8;
+      const h16* bx1 = src + ((prs[1] + t / 3) * WI + pqs[1] + t % 3) * CIPI + g * 8;
+#pragma unroll
+      for (int kc = 0; kc < CIN / 32; ++kc) {
+        half8 b0, b1;
+        __builtin_memcpy(&b0, bx0 + kc * 32, 16);
+        if constexpr (TP == 2) __builtin_memcpy(&b1, bx1 + kc * 32, 16);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          half8 a;
+          __builtin_memcpy(&a, sw + (mt * 16 + r) * KPP + t * CIN + kc * 32 + g * 8, 16);
+          acc[0][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b0, acc[0][mt], 0, 0, 0);
+          if constexpr (TP == 2) acc[1][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b1, acc[1][mt], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TP; ++u) {
+      if (!st[u]) continue;
+      const int pc = pcs[u], pr = prs[u], pq = pqs[u];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const int c0 = mt * 16 + g * 4;
+        if constexpr (LAYER < 3) {
+          half4 ab;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float z = acc[u][mt][q] + bias[c0 + q];
+            ab[q] = (h16)(z * sigmoid(z));
+          }
+          __builtin_memcpy(dst + pc * CIPO + c0, &ab, 8);
+        } else {
+          // core pixel (y0 + pr, x0 + pq); channels c0..c0+3 = fields of fine pixel (py, px)
+          const int y = y0 + pr, x = x0 + pq;
+          const int py = c0 >> 4, pxo = (c0 >> 2) & 3;
+          const int64_t fp = ((int64_t)n * 4 * H + 4 * y + py) * (4 * W) + 4 * x + pxo;
+          float4v sv;
+          __builtin_memcpy(&sv, ap.state + fp * 4, 16);
+          half4 xb;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            sv[q] += (acc[u][mt][q] + bias[c0 + q]) * ap.dscale[q];
+            xb[q] = (h16)((sv[q] - ap.shift[q]) * ap.inv_scale[q]);
+          }
+          xb[3] = (h16)0.f;
+          __builtin_memcpy(ap.state + fp * 4, &sv, 16);
+          __builtin_memcpy(ap.xnext + (((int64_t)n * H + y) * W + x) * 64 + c0, &xb, 8);
+        }
+      }
+    }
+  }
+}
+
+template <int C>
+__global__ void __launch_bounds__(256) fused_step_kernel(const h16* __restrict__ xin, const h16* __restrict__ w0,
+                                                         const h16* __restrict__ w1, const h16* __restrict__ w2,
+                                                         const h16* __restrict__ w3, const float* __restrict__ b0,
+                                                         const float* __restrict__ b1, const float* __restrict__ b2,
+                                                         const float* __restrict__ b3, int N, int H, int W,
+                                                         ApplyArgs ap) {
+  using G = FusedGeom<C>;
+  extern __shared__ __attribute__((aligned(16))) h16 smem[];
+  h16* bufA = smem;
+  h16* bufB = smem + G::BUF_A;
+  h16* sw = smem + G::BUF_A + G::BUF_B;
+  const int tx = W / FT_X, ty = H / FT_Y;
+  const int n = blockIdx.x / (tx * ty);
+  const int y0 = ((blockIdx.x / tx) % ty) * FT_Y, x0 = (blockIdx.x % tx) * FT_X;
+  // layer-0 input region: rows y0 - 4 .. y0 + FT_Y + 3, columns x0 - 4 .. x0 + FT_X + 3 (periodic)
+  {
+    constexpr int N8 = G::R_IN * G::W_IN * 8;
+    const h16* img = xin + (int64_t)n * H * W * 64;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < N8; e += 256) {
+      const int c8 = (e & 7) * 8, q = e >> 3;
+      const int rr = q / G::W_IN, cc = q % G::W_IN;
+      const int yy = (y0 + rr - 4 + 4 * H) % H, xx = (x0 + cc - 4 + 4 * W) % W;
+      half8 v;
+      __builtin_memcpy(&v, img + ((int64_t)yy * W + xx) * 64 + c8, 16);
+      __builtin_memcpy(bufA + q * G::CIP0 + c8, &v, 16);
+    }
+  }
+  fused_copy_w<64, C>(w0, sw);
+  __syncthreads();
+  fused_layer<C, 0, 64, C>(bufA, bufB, sw, b0, n, y0, x0, H, W, ap);
+  __syncthreads();
+  fused_copy_w<C, C>(w1, sw);
+  __syncthreads();
+  fused_layer<C, 1, C, C>(bufB, bufA, sw, b1, n, y0, x0, H, W, ap);
+  __syncthreads();
+  fused_copy_w<C, C>(w2, sw);
+  __syncthreads();
+  fused_layer<C, 2, C, C>(bufA, bufB, sw, b2, n, y0, x0, H, W, ap);
+  __syncthreads();
+  fused_copy_w<C, 64>(w3, sw);
+  __syncthreads();
+  fused_layer<C, 3, C, 64>(bufB, nullptr, sw, b3, n, y0, x0, H, W, ap);
+}
+
 // -------------------------------------------------------- weight gradient
 // Workgroup (bx, tap): partial sums of dW[tap]^T[co][ci] = sum_p dZ[p][co] X[p + d_tap][ci]
 // over the blocks of WG_PIX pixels bx, bx + gridDim.x, ...; the tap-4 (centre) row of
@@ -528,6 +707,25 @@ int conv_launch(const h16* in, const h16* wpk, const float* bias, const h16* zin
   return 0;
 }
 
+template <int C>
+int fused_launch(const lbm_sur_net* net, const h16* xin, h16* xout, const ApplyArgs& ap, int N, int H, int W,
+                        hipStream_t s) {
+  using G = FusedGeom<C>;
+  static_assert(G::LDS <= 160 * 1024, "LDS budget");
+  static bool attr = false;
+  if (!attr) {
+    LBM_CHECK_HIP(hipFuncSetAttribute((const void*)fused_step_kernel<C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)G::LDS));
+    attr = true;
+  }
+  const unsigned blocks = (unsigned)((int64_t)N * (H / FT_Y) * (W / FT_X));
+  fused_step_kernel<C><<<blocks, 256, G::LDS, s>>>(
+      xin, (const h16*)net->wpk[0], (const h16*)net->wpk[1], (const h16*)net->wpk[2], (const h16*)net->wpk[3],
+      net->bias[0], net->bias[1], net->bias[2], net->bias[3], N, H, W, ap);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace
 
 using namespace lbm;
@@ -589,17 +787,29 @@ int lbm_sur_run(const lbm_sur_net* net, float* d_state, const float* dscale, con
                 const float* scale, int64_t nsteps, int N, int H, int W, void* stream) {
   LBM_REQUIRE(net != nullptr && (net->hidden == 32 || net->hidden == 64), "run: hidden must be 32 or 64");
   LBM_REQUIRE(nsteps >= 0, "run: nsteps < 0");
-  const int C = net->hidden;
+  LBM_REQUIRE(N > 0 && H > 0 && W > 0 && H % FT_Y == 0 && W % FT_X == 0,
+              "run: the coarse grid must be a multiple of %dx%d (got %dx%d)", FT_Y, FT_X, H, W);
+  LBM_REQUIRE((int64_t)N * H * W * 64 < ((int64_t)1 << 31), "run: too many pixels");
+  LBM_REQUIRE(net->xalt != nullptr && net->xalt != net->act[0], "run: needs a second input buffer xalt");
+  ApplyArgs ap;
+  ap.dscale = float4v{dscale[0], dscale[1], dscale[2], 0.f};
+  ap.shift = float4v{shift[0], shift[1], shift[2], 0.f};
+  ap.inv_scale = float4v{1.f / scale[0], 1.f / scale[1], 1.f / scale[2], 0.f};
+  ap.state = d_state;
+  hipStream_t s = (hipStream_t)stream;
+  h16* bufs[2] = {(h16*)net->act[0], (h16*)net->xalt};
   for (int64_t k = 0; k < nsteps; ++k) {
-    int rc = lbm_sur_conv(net->act[0], 64, net->wpk[0], net->bias[0], nullptr, net->act[1], nullptr, C, EPI_ACT, N, H,
-                          W, stream);
-    if (!rc) rc = lbm_sur_conv(net->act[1], C, net->wpk[1], net->bias[1], nullptr, net->act[2], nullptr, C, EPI_ACT,
-                               N, H, W, stream);
-    if (!rc) rc = lbm_sur_conv(net->act[2], C, net->wpk[2], net->bias[2], nullptr, net->act[3], nullptr, C, EPI_ACT,
-                               N, H, W, stream);
-    if (!rc) rc = lbm_sur_apply_step(net->act[3], C, net->wpk[3], net->bias[3], d_state, net->act[0], dscale, shift,
-                                     scale, N, H, W, stream);
+    // even steps read act[0] and write xalt, odd steps the reverse
+    const h16* xin = bufs[k & 1];
+    ap.xnext = bufs[(k + 1) & 1];
+    const int rc = net->hidden == 32 ? fused_launch<32>(net, xin, ap.xnext, ap, N, H, W, s)
+                                     : fused_launch<64>(net, xin, ap.xnext, ap, N, H, W, s);
     if (rc) return rc;
+  }
+  if (nsteps & 1) {
+    // the next input must end in act[0]
+    LBM_CHECK_HIP(hipMemcpyAsync(net->act[0], net->xalt, (size_t)N * H * W * 64 * sizeof(h16),
+                                 hipMemcpyDeviceToDevice, s));
   }
   return 0;
 }

@@ -99,8 +99,8 @@ _F = ctypes.c_float
 _FP = ctypes.POINTER(ctypes.c_float)
 
 class LbmSurNet(ctypes.Structure):
-    _fields_ = [("act", ctypes.c_void_p * 4), ("wpk", ctypes.c_void_p * 4), ("bias", ctypes.c_void_p * 4),
-                ("hidden", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+    _fields_ = [("act", ctypes.c_void_p * 4), ("xalt", ctypes.c_void_p), ("wpk", ctypes.c_void_p * 4),
+                ("bias", ctypes.c_void_p * 4), ("hidden", ctypes.c_int32), ("_pad", ctypes.c_int32)]
 
 
 SURROGATE_SIGNATURES = {

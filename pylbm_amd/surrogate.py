@@ -78,6 +78,7 @@ class SurrogateNet:
         self._bufs = []
         al = self._alloc
         self.xin = al(npix * IN_CH * 2)
+        self.xalt = al(self.H * self.W * IN_CH * 2)       # second input buffer of the fused inference step
         self.act = [self.xin] + [al(npix * hidden * 2) for _ in range(3)]
         self.y = al(npix * IN_CH * 4)
         self.w = [al(9 * ch[i] * ch[i + 1] * 4) for i in range(4)]
@@ -196,6 +197,7 @@ class SurrogateNet:
                 c.act[i] = self.act[i]
                 c.wpk[i] = self.afwd[i]
                 c.bias[i] = self.b[i]
+            c.xalt = self.xalt
             c.hidden = self.hidden
             self._cnet = c
         self.rt.check(self.lib.lbm_sur_run(ctypes.byref(self._cnet), ctypes.c_void_p(d_state), dscale, shift, scale,

@@ -156,6 +156,55 @@ def test_make_input_target_and_apply_step(rt):
     net.free()
 
 
+@pytest.mark.parametrize("hidden,steps", [(32, 3), (64, 2), (32, 1)])
+def test_fused_run_matches_layered_steps(rt, hidden, steps):
+    """lbm_sur_run (all layers of a step in one launch, halo recomputed in LDS) against
+    the layer-by-layer path: each path is deterministic, and they agree to the FP16
+    rounding of the hidden activations (a 1-ulp flip of one hidden value moves the
+    state by ~3e-4 of its increment); the next inputs agree to one FP16 ulp."""
+    from pylbm_amd.surrogate import _f3
+    Hf, Wf = 64, 256
+    net, _ = _net(rt, 1, Hf, Wf, hidden=hidden, seed=41, last=1.0)
+    rng = np.random.default_rng(42)
+    s0 = np.zeros((Hf, Wf, 4), np.float32)
+    s0[..., 0] = 1.0 + 1e-3 * rng.normal(size=(Hf, Wf))
+    s0[..., 1:3] = 0.05 * rng.normal(size=(Hf, Wf, 2))
+    na = (_f3((1e-3, 2e-3, 2e-3)), _f3((1.0, 0.0, 0.0)), _f3((2e-3, 0.05, 0.05)))
+    da, db = rt.malloc(s0.nbytes), rt.malloc(s0.nbytes)
+    results = []
+    for d, fused in ((da, False), (db, True), (da, False), (db, True)):
+        rt.h2d(d, s0)
+        rt.check(rt.lib.lbm_sur_make_input(ctypes.c_void_p(d), 1, Hf, Wf, na[1], na[2], ctypes.c_void_p(net.xin),
+                                           rt.stream), "make_input")
+        if fused:
+            net.run(d, na, steps)
+        else:
+            for _ in range(steps):
+                net.apply(d, na)
+        results.append((_download(rt, d, s0.shape, np.float32), _download(rt, net.xin, (net.H, net.W, 64), H16)))
+    (sl, xl), (sf, xf) = results[0], results[1]
+    assert np.array_equal(sl, results[2][0]) and np.array_equal(sf, results[3][0])
+    inc = np.abs(sl.astype(np.float64) - s0)
+    assert inc.max() > 0
+    diff = np.abs(sl.astype(np.float64) - sf)
+    assert diff.max() <= 2e-3 * inc.max(), (diff.max(), inc.max())
+    assert np.mean(diff > 0) < 0.5
+    # each path's next input is the normalised, unshuffled FP16 of its own state, to
+    # one FP16 rounding (the FP32 product is formed in registers, maybe contracted)
+    from oracle import surrogate as osur
+    sh = np.array([1.0, 0.0, 0.0, 0.0], np.float64)
+    isc = 1.0 / np.array([2e-3, 0.05, 0.05, 1.0], np.float64)
+    for st, x in ((sl, xl), (sf, xf)):
+        ref = (st.astype(np.float64) - sh) * isc
+        ref[..., 3] = 0
+        ref = osur.unshuffle(ref[None])[0]
+        err = np.abs(x.astype(np.float64) - ref)
+        assert np.all(err <= 2.0 ** -11 * np.abs(ref) * 1.01 + 1e-7), float((err / (np.abs(ref) + 1e-30)).max())
+    rt.free(da)
+    rt.free(db)
+    net.free()
+
+
 @pytest.mark.parametrize("shape", [(2, 32, 128, 64), (1, 20, 256, 32)])
 def test_gradients_match_oracle(rt, shape):
     """Loss-scaled device gradients / (48 npix) against the oracle's exact gradients."""
