@@ -186,11 +186,15 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in
 // One launch per inference step: a workgroup computes the 4 layers for a core tile
 // of FT_Y x FT_X coarse pixels, recomputing the 3-pixel halo each hidden layer needs
 // (layer l produces a (FT_Y + 2 (3 - l)) x (FT_X + 2 (3 - l)) region).  Activations
-// stay in LDS; each layer's weights are staged in LDS after the previous layer.  The layer-0 input is read from `xin` and the next step's input is
+// stay in LDS.  Input and weights are copied into LDS asynchronously (LDS DMA); with
+// two weight buffers the next layer's weights arrive while the current one computes.  The layer-0 input is read from `xin` and the next step's input is
 // written to `xout` (a different buffer: neighbouring workgroups still read `xin`).
 constexpr int FT_Y = 4, FT_X = 16;
 #ifndef FUSED_PAIR_MASK
 #define FUSED_PAIR_MASK 7  // bit l: layer l of the fused step computes tiles in pairs
+#endif
+#ifndef FUSED_PREFETCH
+#define FUSED_PREFETCH 2   // K steps of LDS operands read ahead of the MFMAs
 #endif
 
 template <int C>
@@ -202,7 +206,11 @@ struct FusedGeom {
   static constexpr int BUF_B = (FT_Y + 6) * (FT_X + 6) * CIPH;
   static constexpr int W0 = C * (9 * 64 + 8), WH = C * (9 * C + 8), W3 = 64 * (9 * C + 8);
   static constexpr int WBUF = W0 > WH ? (W0 > W3 ? W0 : W3) : (WH > W3 ? WH : W3);
-  static constexpr size_t LDS = (size_t)(BUF_A + BUF_B + WBUF) * sizeof(h16);
+  // two weight buffers (layers 0, 2 | 1, 3) when they fit: the next layer's weights
+  // arrive while the current layer computes
+  static constexpr int WB0 = W0 > WH ? W0 : WH, WB1 = WH > W3 ? WH : W3;
+  static constexpr bool DB = (size_t)(BUF_A + BUF_B + WB0 + WB1) * sizeof(h16) <= 160 * 1024;
+  static constexpr size_t LDS = (size_t)(BUF_A + BUF_B + (DB ? WB0 + WB1 : WBUF)) * sizeof(h16);
 };
 
 template <int CIN, int COUT>
@@ -211,18 +219,34 @@ struct LayerW {
   static constexpr int N8 = COUT * (KP / 8);          // 16-byte items
 };
 
-// weights of one layer, global [COUT][9 CIN] -> LDS rows padded to 9 CIN + 8
-template <int CIN, int COUT>
-__device__ __forceinline__ void fused_copy_w(const h16* __restrict__ w, h16* __restrict__ sw) {
-  using L = LayerW<CIN, COUT>;
-  staged_copy<L::N8>(
-      [&](int e) {
-        half8 v;
-        __builtin_memcpy(&v, w + (e / (L::KP / 8)) * L::KP + (e % (L::KP / 8)) * 8, 16);
-        return v;
-      },
-      [&](int e, const half8& v) { __builtin_memcpy(sw + (e / (L::KP / 8)) * L::KPP + (e % (L::KP / 8)) * 8, &v, 16); });
+// Asynchronous copy into LDS (gfx950 global_load_lds_dwordx4): NU 16-byte units,
+// unit u lands at dst + 8 u halves and is read from src_of(u).  A wave issues the
+// units [64 k, 64 k + 64) for k = wave, wave + 4, ...; the caller waits
+// (s_waitcnt + barrier) before reading dst.
+template <int NU, typename F>
+__device__ __forceinline__ void dma_units(h16* dst, F src_of) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int k = wv; k * 64 < NU; k += 4) {
+    const int u = k * 64 + lane;
+    if (u < NU)
+      __builtin_amdgcn_global_load_lds((const void*)src_of(u),
+                                       (__attribute__((address_space(3))) void*)(dst + k * 64 * 8), 16, 0, 0);
+  }
 }
+
+// weights of one layer, global [COUT][9 CIN] -> LDS rows padded to 9 CIN + 8 halves
+// (the last unit of a row is padding: it receives a copy of the row's last unit)
+template <int CIN, int COUT>
+__device__ __forceinline__ void dma_weights(const h16* __restrict__ w, h16* sw) {
+  constexpr int UR = 9 * CIN / 8 + 1;
+  dma_units<COUT * UR>(sw, [&](int u) {
+    const int row = u / UR, c = u % UR;
+    return w + row * (9 * CIN) + (c < UR - 1 ? c : UR - 2) * 8;
+  });
+}
+
+__device__ __forceinline__ void dma_wait() { __builtin_amdgcn_s_waitcnt(0); }
 
 // Layer LAYER of the tile: input region (RO + 2) x (WO + 2) at `src` (stride CIPI),
 // output region RO x WO at `dst` (stride CIPO), or the state / next input for LAYER 3.
@@ -260,22 +284,33 @@ __device__ __forceinline__ void fused_layer(const h16* __restrict__ src, h16* __
     for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) acc[u][mt] = float4v{0.f, 0.f, 0.f, 0.f};
+    // K steps ks = t * KC + kc, software-pipelined: the operands of step ks + 1 are
+    // read from LDS while the MFMAs of step ks issue
+    constexpr int KC = CIN / 32, NK = 9 * KC;
+    const h16* bbase0 = src + (prs[0] * WI + pqs[0]) * CIPI + g * 8;
+    const h16* bbase1 = src + (prs[1] * WI + pqs[1]) * CIPI + g * 8;
+    const h16* abase = sw + r * KPP + g * 8;
+    // ring of FUSED_PREFETCH + 1 operand sets
+    constexpr int D = FUSED_PREFETCH;
+    half8 bop[D + 1][2], aop[D + 1][MT];
+    auto load_ops = [&](int ks, half8 (&b)[2], half8 (&a)[MT]) {
+      const int t = ks / KC, kc = ks % KC;
+      const int boff = ((t / 3) * WI + t % 3) * CIPI + kc * 32;
+      __builtin_memcpy(&b[0], bbase0 + boff, 16);
+      if constexpr (TP == 2) __builtin_memcpy(&b[1], bbase1 + boff, 16);
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-      const h16* bx0 = src + ((prs[0] + t / 3) * WI + pqs[0] + t % 3) * CIPI + g * 8;
-      const h16* bx1 = src + ((prs[1] + t / 3) * WI + pqs[1] + t % 3) * CIPI + g * 8;
+      for (int mt = 0; mt < MT; ++mt) __builtin_memcpy(&a[mt], abase + mt * 16 * KPP + t * CIN + kc * 32, 16);
+    };
 #pragma unroll
-      for (int kc = 0; kc < CIN / 32; ++kc) {
-        half8 b0, b1;
-        __builtin_memcpy(&b0, bx0 + kc * 32, 16);
-        if constexpr (TP == 2) __builtin_memcpy(&b1, bx1 + kc * 32, 16);
+    for (int ks = 0; ks < D && ks < NK; ++ks) load_ops(ks, bop[ks], aop[ks]);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          half8 a;
-          __builtin_memcpy(&a, sw + (mt * 16 + r) * KPP + t * CIN + kc * 32 + g * 8, 16);
-          acc[0][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b0, acc[0][mt], 0, 0, 0);
-          if constexpr (TP == 2) acc[1][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b1, acc[1][mt], 0, 0, 0);
-        }
+    for (int ks = 0; ks < NK; ++ks) {
+      if (ks + D < NK) load_ops(ks + D, bop[(ks + D) % (D + 1)], aop[(ks + D) % (D + 1)]);
+      const int c = ks % (D + 1);
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        acc[0][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aop[c][mt], bop[c][0], acc[0][mt], 0, 0, 0);
+        if constexpr (TP == 2) acc[1][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aop[c][mt], bop[c][1], acc[1][mt], 0, 0, 0);
       }
     }
 #pragma unroll
@@ -326,39 +361,54 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const h16* __restrict__
   extern __shared__ __attribute__((aligned(16))) h16 smem[];
   h16* bufA = smem;
   h16* bufB = smem + G::BUF_A;
-  h16* sw = smem + G::BUF_A + G::BUF_B;
+  h16* sw0 = smem + G::BUF_A + G::BUF_B;
+  h16* sw1 = G::DB ? sw0 + G::WB0 : sw0;
   const int tx = W / FT_X, ty = H / FT_Y;
   const int n = blockIdx.x / (tx * ty);
   const int y0 = ((blockIdx.x / tx) % ty) * FT_Y, x0 = (blockIdx.x % tx) * FT_X;
-  // layer-0 input region: rows y0 - 4 .. y0 + FT_Y + 3, columns x0 - 4 .. x0 + FT_X + 3 (periodic)
+  // layer-0 input region: rows y0 - 4 .. y0 + FT_Y + 3, columns x0 - 4 .. x0 + FT_X + 3
+  // (periodic), 9 units per pixel (8 of channels + 1 of padding)
   {
-    constexpr int N8 = G::R_IN * G::W_IN * 8;
     const h16* img = xin + (int64_t)n * H * W * 64;
-#pragma unroll 4
-    for (int e = threadIdx.x; e < N8; e += 256) {
-      const int c8 = (e & 7) * 8, q = e >> 3;
+    dma_units<G::R_IN * G::W_IN * 9>(bufA, [&](int u) {
+      const int q = u / 9, c = u % 9;
       const int rr = q / G::W_IN, cc = q % G::W_IN;
       const int yy = (y0 + rr - 4 + 4 * H) % H, xx = (x0 + cc - 4 + 4 * W) % W;
-      half8 v;
-      __builtin_memcpy(&v, img + ((int64_t)yy * W + xx) * 64 + c8, 16);
-      __builtin_memcpy(bufA + q * G::CIP0 + c8, &v, 16);
-    }
+      return img + ((int64_t)yy * W + xx) * 64 + (c < 8 ? c : 7) * 8;
+    });
   }
-  fused_copy_w<64, C>(w0, sw);
+  dma_weights<64, C>(w0, sw0);
+  if constexpr (G::DB) dma_weights<C, C>(w1, sw1);
+  dma_wait();
   __syncthreads();
-  fused_layer<C, 0, 64, C>(bufA, bufB, sw, b0, n, y0, x0, H, W, ap);
+  fused_layer<C, 0, 64, C>(bufA, bufB, sw0, b0, n, y0, x0, H, W, ap);
   __syncthreads();
-  fused_copy_w<C, C>(w1, sw);
+  if constexpr (G::DB) {
+    dma_weights<C, C>(w2, sw0);
+  } else {
+    dma_weights<C, C>(w1, sw0);
+    dma_wait();
+    __syncthreads();
+  }
+  fused_layer<C, 1, C, C>(bufB, bufA, sw1, b1, n, y0, x0, H, W, ap);
+  dma_wait();
   __syncthreads();
-  fused_layer<C, 1, C, C>(bufB, bufA, sw, b1, n, y0, x0, H, W, ap);
+  if constexpr (G::DB) {
+    dma_weights<C, 64>(w3, sw1);
+  } else {
+    dma_weights<C, C>(w2, sw0);
+    dma_wait();
+    __syncthreads();
+  }
+  fused_layer<C, 2, C, C>(bufA, bufB, sw0, b2, n, y0, x0, H, W, ap);
+  dma_wait();
   __syncthreads();
-  fused_copy_w<C, C>(w2, sw);
-  __syncthreads();
-  fused_layer<C, 2, C, C>(bufA, bufB, sw, b2, n, y0, x0, H, W, ap);
-  __syncthreads();
-  fused_copy_w<C, 64>(w3, sw);
-  __syncthreads();
-  fused_layer<C, 3, C, 64>(bufB, nullptr, sw, b3, n, y0, x0, H, W, ap);
+  if constexpr (!G::DB) {
+    dma_weights<C, 64>(w3, sw0);
+    dma_wait();
+    __syncthreads();
+  }
+  fused_layer<C, 3, C, 64>(bufB, nullptr, sw1, b3, n, y0, x0, H, W, ap);
 }
 
 // -------------------------------------------------------- weight gradient
