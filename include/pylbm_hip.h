@@ -104,6 +104,7 @@ int lbm_event_destroy(void* event);
 int lbm_event_record(void* event, void* stream);
 int lbm_event_elapsed_ms(void* start, void* stop, float* ms);
 int lbm_event_sync(void* event);                           /* wait for the event on the host */
+int lbm_stream_wait_event(void* stream, void* event);     /* later work on stream waits for event */
 /* pinned host memory: the target of copies the host reads without draining the stream */
 int lbm_host_malloc(void** h_ptr, uint64_t bytes);
 int lbm_host_free(void* h_ptr);

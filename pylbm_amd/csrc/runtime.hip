@@ -414,6 +414,11 @@ int lbm_event_sync(void* event) {
   return 0;
 }
 
+int lbm_stream_wait_event(void* stream, void* event) {
+  LBM_CHECK_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+  return 0;
+}
+
 int lbm_host_malloc(void** h_ptr, uint64_t bytes) {
   LBM_REQUIRE(h_ptr, "h_ptr is NULL");
   *h_ptr = nullptr;

@@ -85,6 +85,7 @@ CORE_SIGNATURES = {
     "lbm_event_record": ([_P, _P], _I),
     "lbm_event_elapsed_ms": ([_P, _P, ctypes.POINTER(ctypes.c_float)], _I),
     "lbm_event_sync": ([_P], _I),
+    "lbm_stream_wait_event": ([_P, _P], _I),
     "lbm_host_malloc": ([ctypes.POINTER(_P), _U64], _I),
     "lbm_host_free": ([_P], _I),
     "lbm_ghost_fill": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmVelocities), _P], _I),

@@ -69,6 +69,7 @@ class SurrogateNet:
         if Hf % PATCH or Wf % (PATCH * 32):
             raise ValueError("the surrogate needs Hf % 4 == 0 and Wf % 128 == 0 (got %dx%d)" % (Hf, Wf))
         self.rt, self.lib, self.batch, self.train = rt, rt.lib, batch, train
+        self.stream = rt.stream  # every launch of the network goes to this stream
         self.Hf, self.Wf = Hf, Wf
         self.H, self.W = Hf // PATCH, Wf // PATCH
         self.hidden = hidden
@@ -111,7 +112,7 @@ class SurrogateNet:
         return p
 
     def _zero(self, ptr, nbytes):
-        self.rt.check(self.lib.lbm_memset(ctypes.c_void_p(ptr), 0, int(nbytes), self.rt.stream), "memset")
+        self.rt.check(self.lib.lbm_memset(ctypes.c_void_p(ptr), 0, int(nbytes), self.stream), "memset")
 
     def free(self):
         for p in self._bufs:
@@ -152,14 +153,14 @@ class SurrogateNet:
             abwd = self.abwd[i] if self.train else None
             self.rt.check(self.lib.lbm_sur_pack(ctypes.c_void_p(self.w[i]), ch[i], ch[i + 1], ch[i], ch[i + 1],
                                                 ctypes.c_void_p(self.afwd[i]), ctypes.c_void_p(abwd),
-                                                self.rt.stream), "lbm_sur_pack")
+                                                self.stream), "lbm_sur_pack")
 
     # ------------------------------------------------------------ passes
     def _conv(self, src, cinp, wpk, bias, zin, out, zout, coutp, epi, n):
         self.rt.check(self.lib.lbm_sur_conv(ctypes.c_void_p(src), cinp, ctypes.c_void_p(wpk),
                                             ctypes.c_void_p(bias), ctypes.c_void_p(zin), ctypes.c_void_p(out),
                                             ctypes.c_void_p(zout), coutp, epi, n, self.H, self.W,
-                                            self.rt.stream), "lbm_sur_conv")
+                                            self.stream), "lbm_sur_conv")
 
     def hidden_layers(self, n=1, keep=False):
         """Layers 0..2 from xin; keep=True stores the pre-activations for backward."""
@@ -185,7 +186,7 @@ class SurrogateNet:
         self.rt.check(self.lib.lbm_sur_apply_step(ctypes.c_void_p(self.act[3]), self.hidden,
                                                   ctypes.c_void_p(self.afwd[3]), ctypes.c_void_p(self.b[3]),
                                                   ctypes.c_void_p(d_state), ctypes.c_void_p(self.xin), dscale,
-                                                  shift, scale, n, self.H, self.W, self.rt.stream),
+                                                  shift, scale, n, self.H, self.W, self.stream),
                       "lbm_sur_apply_step")
 
     def run(self, d_state, norm_args, nsteps):
@@ -201,19 +202,19 @@ class SurrogateNet:
             c.hidden = self.hidden
             self._cnet = c
         self.rt.check(self.lib.lbm_sur_run(ctypes.byref(self._cnet), ctypes.c_void_p(d_state), dscale, shift, scale,
-                                           int(nsteps), 1, self.H, self.W, self.rt.stream), "lbm_sur_run")
+                                           int(nsteps), 1, self.H, self.W, self.stream), "lbm_sur_run")
 
     def loss(self, n, gscale=1.0, sync=True):
         """Masked MSE of y against tgt; writes dY = 2 gscale (y - t). Returns the loss if sync."""
         out = ctypes.c_double(0.0)
         self.rt.check(self.lib.lbm_sur_mse(ctypes.c_void_p(self.y), ctypes.c_void_p(self.tgt), n * self.H * self.W,
                                            float(gscale), ctypes.c_void_p(self.dy), ctypes.c_void_p(self.lpart),
-                                           ctypes.byref(out) if sync else None, self.rt.stream), "lbm_sur_mse")
+                                           ctypes.byref(out) if sync else None, self.stream), "lbm_sur_mse")
         return out.value if sync else None
 
     def backward(self, n):
         """Weight and bias gradients (dwt, db) of every layer from dY."""
-        lib, st, ch = self.lib, self.rt.stream, self.ch
+        lib, st, ch = self.lib, self.stream, self.ch
         grad_in = self.dy
         for i in range(3, -1, -1):
             self.rt.check(lib.lbm_sur_wgrad(ctypes.c_void_p(self.act[i]), ch[i], ctypes.c_void_p(grad_in), ch[i + 1],
@@ -234,7 +235,7 @@ class SurrogateNet:
                                                 ctypes.c_void_p(self.m[i]), ctypes.c_void_p(self.v[i]),
                                                 ctypes.c_void_p(self.dwt[i]), ctypes.c_void_p(self.db[i]),
                                                 ch[i], ch[i + 1], ch[i], ch[i + 1], float(lr), float(b1), float(b2),
-                                                float(eps * eps_scale), self.step, self.rt.stream), "lbm_sur_adam")
+                                                float(eps * eps_scale), self.step, self.stream), "lbm_sur_adam")
         self.pack()
 
     def train_step(self, n, lr, sync=False):
@@ -314,21 +315,50 @@ class LatticeState:
                                                         self.iqx, self.iqy, self.H, self.W, ctypes.c_void_p(d_state),
                                                         self.rt.stream), "lbm_sur_moments_to_state")
 
+    def _bind(self):
+        """ctypes arguments of the judge path, built once (the buffers of F, Fnew and m
+        do not move; F / Fnew swap only for odd step counts, which run() forbids)."""
+        sim = self.sim
+        c = sim.container
+        m = c.m
+        self._m_ptr, self._m_lay = ctypes.c_void_p(m.ptr), ctypes.byref(m.layout)
+        self._f_lay = ctypes.byref(c.F.layout)
+        self._params = ctypes.byref(sim.params)
+        self._vel = ctypes.byref(sim.vel)
+        self._parity = ctypes.c_int(0)
+        self._bound = (c.F.ptr, c.Fnew.ptr)
+
     def restart(self, d_state):
         """Populations from a state: conserved moments -> equilibrium -> m2f (pylbm moment init)."""
         sim = self.sim
-        m = sim.container.m
-        self.rt.check(self.lib.lbm_sur_state_to_moments(ctypes.c_void_p(d_state), self.H, self.W,
-                                                        ctypes.byref(m.layout), self.irho, self.iqx, self.iqy,
-                                                        ctypes.c_void_p(m.ptr), self.rt.stream),
-                      "lbm_sur_state_to_moments")
-        sim._equilibrium(m)
-        sim._m2f(m, sim.container.F)
+        c = sim.container
+        if getattr(self, "_bound", None) != (c.F.ptr, c.Fnew.ptr):
+            self._bind()
+        st, lib, mod = self.rt.stream, self.lib, sim.module
+        self.rt.check(lib.lbm_sur_state_to_moments(ctypes.c_void_p(d_state), self.H, self.W, self._m_lay, self.irho,
+                                                   self.iqx, self.iqy, self._m_ptr, st), "lbm_sur_state_to_moments")
+        self.rt.check(mod.lbm_gen_equilibrium(self._m_ptr, self._m_lay, st), "equilibrium")
+        self.rt.check(mod.lbm_gen_m2f(self._m_ptr, self._m_lay, ctypes.c_void_p(c.F.ptr), self._f_lay, self._params,
+                                      st), "m2f")
         # the fold step writes every cell of Fnew and reads no halo of F
         sim._update_m = True
 
     def run(self, nsteps):
-        self.sim.run(nsteps)
+        """nsteps plain steps (fully periodic: the folded fused kernel, no links)."""
+        if nsteps % 2:
+            self.sim.run(nsteps)
+            return
+        sim = self.sim
+        c = sim.container
+        if getattr(self, "_bound", None) != (c.F.ptr, c.Fnew.ptr):
+            self._bind()
+        sim.params.t = sim.t
+        self.rt.check(sim.module.lbm_gen_run(ctypes.c_void_p(c.F.ptr), ctypes.c_void_p(c.Fnew.ptr), self._f_lay,
+                                             self._vel, self._params, sim._link_sets, 0, int(nsteps), 0, 1,
+                                             ctypes.byref(self._parity), self.rt.stream), "run")
+        sim._update_m = True
+        sim.nt += nsteps
+        sim.t = sim.t + nsteps * sim.dt
 
     def initial_state(self, seed, rms=0.05, kmin=4.0, kmax=16.0):
         """Host FP32 state of the random Fourier velocity of ``seed`` (rho = 1)."""
@@ -461,15 +491,25 @@ class Trainer:
 class SurrogateRun:
     """Advance a LatticeState by intervals of K steps with the network, judged by the plain run."""
 
-    def __init__(self, net, lattice, norm, K=16, judge_every=8, tol=1e-2):
+    def __init__(self, net, lattice, norm, K=16, judge_every=8, tol=1e-2, overlap=False):
         if net.Hf != lattice.H or net.Wf != lattice.W:
             raise ValueError("network and lattice sizes differ")
         self.net, self.lat, self.norm = net, lattice, norm
         self.K, self.judge_every, self.tol = K, judge_every, tol
+        self.overlap = overlap
         self.rt, self.lib = lattice.rt, lattice.lib
         self.state = self.rt.malloc(lattice.npix * 16)
         self.pred = self.rt.malloc(lattice.npix * 16)
         self.ckpt = [self.rt.malloc(lattice.npix * 16) for _ in range(3)]
+        # overlapped judging: the network runs on its own stream; per judge parity the
+        # judge's input state and the network's prediction of that interval
+        self.jin = [self.rt.malloc(lattice.npix * 16) for _ in range(2)]
+        self.jpred = [self.rt.malloc(lattice.npix * 16) for _ in range(2)]
+        self.jout = self.rt.malloc(lattice.npix * 16)
+        sa = ctypes.c_void_p()
+        self.rt.check(self.lib.lbm_stream_create(ctypes.byref(sa)), "lbm_stream_create")
+        self.net_stream = sa
+        self.ev = {k: [self.rt.event(), self.rt.event()] for k in ("in", "pred", "free", "drift")}
         self.lpart = self.rt.malloc(514 * 8)
         hp = ctypes.c_void_p()
         self.rt.check(self.lib.lbm_host_malloc(ctypes.byref(hp), 2 * 2 * 8), "lbm_host_malloc")
@@ -480,10 +520,17 @@ class SurrogateRun:
         self.fallback_at = None
 
     def free(self):
-        for p in [self.state, self.pred, self.lpart] + self.ckpt:
+        for p in [self.state, self.pred, self.lpart, self.jout] + self.ckpt + self.jin + self.jpred:
             self.rt.free(p)
-        self.state = self.pred = self.lpart = None
-        self.ckpt = []
+        self.state = self.pred = self.lpart = self.jout = None
+        self.ckpt, self.jin, self.jpred = [], [], []
+        for pair in self.ev.values():
+            for e in pair:
+                self.lib.lbm_event_destroy(e)
+        self.ev = {}
+        if self.net_stream:
+            self.lib.lbm_stream_destroy(self.net_stream)
+            self.net_stream = None
         if self.hdrift:
             self.rt.check(self.lib.lbm_host_free(ctypes.c_void_p(self.hdrift)), "lbm_host_free")
             self.hdrift = None
@@ -525,7 +572,108 @@ class SurrogateRun:
         d, n = (ctypes.c_double * 2).from_address(self.hdrift + 16 * slot)
         return 0.0 if d == 0.0 and n == 0.0 else float(np.sqrt(d / n)) if n > 0 else float("nan")
 
+    # ------------------------------------------------------------ helpers
+    def _copy(self, dst, src, stream):
+        self.rt.check(self.lib.lbm_memcpy_d2d(ctypes.c_void_p(dst), ctypes.c_void_p(src), self.lat.npix * 16,
+                                              stream), "lbm_memcpy_d2d")
+
+    def _record(self, event, stream):
+        self.rt.check(self.lib.lbm_event_record(event, stream), "lbm_event_record")
+
+    def _wait(self, stream, event):
+        self.rt.check(self.lib.lbm_stream_wait_event(stream, event), "lbm_stream_wait_event")
+
     def advance(self, nintervals):
+        """nintervals intervals of K lattice steps (see the class docstring); the result
+        ends in self.state and in the lattice."""
+        if self.overlap:
+            self._advance_overlapped(nintervals)
+        else:
+            self._advance_anchored(nintervals)
+
+    def _advance_overlapped(self, nintervals):
+        """The network runs every interval on its own stream; every judge_every-th
+        interval the lattice (on its stream) re-runs that interval from the same
+        state, concurrently with the network's next intervals.  Judge j's lattice
+        result is checkpoint j % 3; its drift is read on the host after judge j + 1
+        is queued.  A failed judge returns to the last checkpoint whose judge passed
+        (or the start) and runs the plain lattice method from there to the end."""
+        lat, K, je = self.lat, self.K, self.judge_every
+        A, B = self.net_stream, self.rt.stream
+        norm_args = (self._dscale, self._shift, self._scale)
+        ev = self.ev
+        saved = self.net.stream
+        self.net.stream = A
+        confirmed, confirmed_at = 2, 0
+        self.rt.d2d(self.ckpt[confirmed], self.state, lat.npix * 16)   # on B, before A starts
+        self._record(ev["free"][0], B)
+        self._wait(A, ev["free"][0])
+        self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(self.state), 1, lat.H, lat.W, self._shift,
+                                                  self._scale, ctypes.c_void_p(self.net.xin), A), "make_input")
+        pending = None
+        i = j = 0
+
+        def check(p):
+            drift = self._drift_wait(p[0] % 2)
+            self.judges.append(drift)
+            return drift <= self.tol  # False for NaN as well
+
+        def fall_back(at):
+            self.fallback_at = at
+            self.rt.check(self.lib.lbm_stream_sync(A), "lbm_stream_sync")
+            self.rt.d2d(self.state, self.ckpt[confirmed], lat.npix * 16)
+            lat.restart(self.state)
+            lat.run(K * (nintervals - confirmed_at))
+            lat.read_state(self.state)
+
+        try:
+            while i < nintervals:
+                nsur = min(je - 1 - (i % je), nintervals - i)
+                if nsur > 0:
+                    self.net.run(self.state, norm_args, nsur)
+                    i += nsur
+                    continue
+                q = j % 2
+                if j >= 2:
+                    self._wait(A, ev["free"][q])           # B is done with jin[q], jpred[q]
+                self._copy(self.jin[q], self.state, A)
+                self._record(ev["in"][q], A)
+                self.net.run(self.state, norm_args, 1)   # the network's own step of the judged interval
+                self._copy(self.jpred[q], self.state, A)
+                self._record(ev["pred"][q], A)
+                # the lattice re-runs the interval on B
+                self._wait(B, ev["in"][q])
+                lat.restart(self.jin[q])
+                lat.run(K)
+                lat.read_state(self.jout)
+                self._wait(B, ev["pred"][q])
+                self.rt.check(self.lib.lbm_sur_rel_l2_async(ctypes.c_void_p(self.jpred[q]), ctypes.c_void_p(self.jout),
+                                                            lat.npix, ctypes.c_void_p(self.lpart),
+                                                            ctypes.c_void_p(self.hdrift + 16 * q), B),
+                              "lbm_sur_rel_l2_async")
+                self._record(self.events[q], B)
+                self._copy(self.ckpt[j % 3], self.jout, B)
+                self._record(ev["free"][q], B)
+                i += 1
+                if pending is not None:
+                    if not check(pending):
+                        fall_back(pending[1] - 1)
+                        return
+                    confirmed, confirmed_at = pending[0] % 3, pending[1]
+                pending = (j, i)
+                j += 1
+            if pending is not None and not check(pending):
+                fall_back(pending[1] - 1)
+                return
+            # the lattice takes the final state
+            self._record(ev["in"][0], A)
+            self._wait(B, ev["in"][0])
+            lat.restart(self.state)
+        finally:
+            self.rt.check(self.lib.lbm_stream_sync(A), "lbm_stream_sync")
+            self.net.stream = saved
+
+    def _advance_anchored(self, nintervals):
         """nintervals intervals of K lattice steps; state ends in self.state (and in the lattice).
 
         Every judge_every-th interval is computed by the plain lattice run from the

@@ -2,7 +2,9 @@
 on the CPU: device buffers, the network and the lattice are replaced by a model
 that tracks which simulated interval each buffer holds and by whom it was
 produced, so the scheduling of judges, checkpoints, deferred drift reads and the
-fall-back can be checked exactly."""
+fall-back can be checked exactly, for the anchored and the overlapped schedule."""
+import ctypes
+
 import pytest
 
 from pylbm_amd.surrogate import SurrogateRun
@@ -54,6 +56,7 @@ def _run(nintervals, je, bad=()):
     bufs = {}
     log = []
     run = object.__new__(SurrogateRun)
+    run.overlap = False
     run.K, run.judge_every, run.tol = 16, je, 1e-2
     run.rt = _Rt(bufs)
     run.net = _Net(bufs, log)
@@ -101,4 +104,103 @@ def test_failed_judge_falls_back_to_last_passed_checkpoint(nint, je, bad):
     assert t == nint
     prefix = "".join("L" if (i + 1) % je == 0 else "s" for i in range(start))
     assert hist == prefix + "L" * (nint - start)
+    assert len(run.judges) == bad + 1
+
+
+# ------------------------------------------------------------ overlapped schedule
+def _v(x):
+    return x.value if isinstance(x, ctypes.c_void_p) else x
+
+
+class _Lib:
+    """The entry points the overlapped schedule calls, on the buffer model."""
+
+    def __init__(self, bufs, judged):
+        self.bufs, self.judged = bufs, judged
+
+    def lbm_memcpy_d2d(self, dst, src, nbytes, stream):
+        self.bufs[_v(dst)] = self.bufs[_v(src)]
+        return 0
+
+    def lbm_sur_rel_l2_async(self, a, b, npix, part, host, stream):
+        self.judged.append(((_v(host) - 1000) // 16, self.bufs[_v(b)]))
+        return 0
+
+    def lbm_sur_make_input(self, *args):
+        return 0
+
+    def lbm_event_record(self, e, s):
+        return 0
+
+    def lbm_stream_wait_event(self, s, e):
+        return 0
+
+    def lbm_stream_sync(self, s):
+        return 0
+
+
+class _RtO(_Rt):
+    def check(self, rc, what):
+        assert rc == 0, what
+
+
+def _run_overlapped(nintervals, je, bad=()):
+    bufs = {}
+    log = []
+    judged = []
+    run = object.__new__(SurrogateRun)
+    run.overlap = True
+    run.K, run.judge_every, run.tol = 16, je, 1e-2
+    run.rt = _RtO(bufs)
+    run.rt.stream = "B"
+    run.lib = _Lib(bufs, judged)
+    run.net = _Net(bufs, log)
+    run.net.stream = "B"
+    run.net.xin = 0
+    run.lat = _Lat(bufs, 16, log)
+    run.lat.H = run.lat.W = 4
+    run.state, run.pred, run.lpart, run.jout, run.hdrift = 1, 2, 50, 40, 1000
+    run.ckpt, run.jin, run.jpred = [10, 11, 12], [20, 21], [30, 31]
+    run.net_stream = "A"
+    run.ev = {k: [None, None] for k in ("in", "pred", "free", "drift")}
+    run.events = [None, None]
+    run._dscale = run._shift = run._scale = None
+    run.judges, run.fallback_at = [], None
+
+    def drift_wait(slot):
+        k = max(i for i, (s, _) in enumerate(judged) if s == slot)
+        return 1.0 if k in bad else 1e-3
+
+    run._drift_wait = drift_wait
+    bufs[1] = (0, "")
+    run.advance(nintervals)
+    assert run.net.stream == "B"
+    return run, bufs, judged
+
+
+@pytest.mark.parametrize("nint,je", [(16, 8), (125, 8), (7, 8), (13, 4), (1, 1), (3, 1)])
+def test_overlapped_all_judges_pass(nint, je):
+    run, bufs, judged = _run_overlapped(nint, je)
+    t, hist = bufs[1]
+    # the network does every interval; the lattice only re-ran the judged ones
+    assert t == nint and hist == "s" * nint and run.fallback_at is None
+    assert len(run.judges) == nint // je
+    # judge k compared the lattice re-run of interval (k + 1) je - 1 from the network state
+    for k, (slot, (tj, hj)) in enumerate(judged):
+        assert slot == k % 2 and tj == (k + 1) * je and hj == "s" * ((k + 1) * je - 1) + "L"
+
+
+@pytest.mark.parametrize("nint,je,bad", [(64, 8, 0), (64, 8, 1), (64, 8, 3), (64, 8, 7), (20, 4, 2), (5, 1, 0),
+                                         (6, 1, 4)])
+def test_overlapped_failed_judge_falls_back_to_last_passed_checkpoint(nint, je, bad):
+    run, bufs, judged = _run_overlapped(nint, je, bad=(bad,))
+    t, hist = bufs[1]
+    assert run.fallback_at == (bad + 1) * je - 1
+    assert t == nint
+    # the last passed judge (bad - 1) holds the lattice re-run of its interval
+    if bad == 0:
+        prefix = ""
+    else:
+        prefix = "s" * (bad * je - 1) + "L"
+    assert hist == prefix + "L" * (nint - len(prefix))
     assert len(run.judges) == bad + 1
