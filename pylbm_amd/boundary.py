@@ -276,7 +276,7 @@ class DeviceLinks:
 
 # ------------------------------------------------------ ghost slots outside the fill
 def unwritten_ghost_reads(domain, stencil, methods, layout):
-    """Ghost slots a pull reads that neither the periodic-face fill nor a link list writes.
+    """Ghost slots a pull or a Neumann-type list reads that the periodic-face fill does not write.
 
     pylbm wraps every halo slot before the link lists run (pylbm/storage.py:306-367).
     The device fill only wraps the slabs of periodic faces (a slot belongs to the
@@ -321,6 +321,28 @@ def unwritten_ghost_reads(domain, stencil, methods, layout):
                 if keep.any():
                     stores.append(st[keep])
                     loads.append(_flat(np.concatenate([kk, wrapped]), layout)[keep])
+    # loads of the Neumann-type lists that fall on ghost slots: pylbm's wrap put the
+    # fully wrapped value there before any list ran (its axis-by-axis wrap ends every
+    # slot with a copy of an already wrapped one); a slot an earlier list stores to is
+    # overwritten after this copy, as in pylbm
+    for m in methods:
+        if m.kind.shift_axis is None and m.kind.symmetric_load:
+            continue
+        rows = np.asarray(m.iload[0].T if m._fixed else m.iload[0])
+        cells = rows[1:]
+        ghost = np.any([(cells[b] < h[b]) | (cells[b] >= shape[b] - h[b]) for b in range(dim)], axis=0)
+        if not ghost.any():
+            continue
+        s_ = cells[:, ghost]
+        wrapped = s_.copy()
+        for b in range(dim):
+            nin = shape[b] - 2 * h[b]
+            wrapped[b] = np.where(s_[b] < h[b], s_[b] + nin, np.where(s_[b] >= shape[b] - h[b], s_[b] - nin, s_[b]))
+        kk = rows[:1, ghost]
+        stores.append(_flat(np.concatenate([kk, s_]), layout))
+        loads.append(_flat(np.concatenate([kk, wrapped]), layout))
     if not stores:
         return np.zeros(0, np.int64), np.zeros(0, np.int64)
-    return np.concatenate(stores), np.concatenate(loads)
+    st, ld = np.concatenate(stores), np.concatenate(loads)
+    st, first = np.unique(st, return_index=True)
+    return st, ld[first]

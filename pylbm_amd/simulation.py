@@ -147,8 +147,10 @@ class Simulation:
         for method in self.bc.methods:
             method.set_iload()
         self._dev_links = []
-        self._ghost_mode = 2 if any(m.kind.shift_axis is not None or not m.kind.symmetric_load
-                                    for m in self.bc.methods) else 1
+        # ghost slots: the periodic-face slabs are filled on the device and the other
+        # slots any pull or Neumann-type list reads come from a host-computed copy list
+        # (boundary.unwritten_ghost_reads); the full wrap (2) stays available
+        self._ghost_mode = 1
         self.init_type = dico.get("inittype", "moments")
         self.init_data = dico.get("init", None)
         self._need_init = True

@@ -7,7 +7,7 @@ import pytest
 from oracle.boundary import build_links
 from oracle.domain import Domain as ODomain
 from oracle.lbm import method_kind
-from pylbm_amd import cases
+from pylbm_amd import bc, cases
 from pylbm_amd.boundary import Boundary, _resolve_sequential
 from pylbm_amd.domain import Domain
 
@@ -168,12 +168,26 @@ def _emulated_step(o, prod_methods, dom):
     return F, st.size
 
 
-@pytest.mark.parametrize("name", ["spheres", "lid", "d1q3"])
+def _neumann_periodic():
+    """D2Q9 with NeumannX on both x faces and periodic y: the corner links load ghost
+    slots of the periodic y slabs that the entering-population fill does not write."""
+    d = cases.karman_d2q9(24, 12, radius=2.0, center=(8.0, 6.0))
+    d["box"] = {"x": [0.0, 24.0], "y": [0.0, 12.0], "label": [2, 2, -1, -1]}
+    d["elements"] = []
+    d["boundary_conditions"] = {2: {"method": {0: bc.NeumannX}}}
+    return d
+
+
+@pytest.mark.parametrize("name", ["spheres", "lid", "d1q3", "d1q3_walls", "karman", "vectorial", "neumann_periodic"])
 def test_restricted_ghost_fill_equals_full_wrap(name):
     from oracle.lbm import Simulation as O
     d = {"spheres": lambda: cases.sphere_packing_d3q19(20, 14, 12, seed=3, solid_fraction=0.25),
          "lid": lambda: cases.lid_cavity_d2q9(12),
-         "d1q3": lambda: cases.d1q3_advection(10, walls=False)}[name]()
+         "d1q3": lambda: cases.d1q3_advection(10, walls=False),
+         "d1q3_walls": lambda: cases.d1q3_advection(10, walls=True),
+         "karman": lambda: cases.karman_d2q9(40, 20, radius=4.0, center=(10.0, 10.0)),
+         "vectorial": lambda: cases.d2q5q9_vectorial(12),
+         "neumann_periodic": _neumann_periodic}[name]()
     o = O(d)
     for _ in range(2):
         o.one_time_step()
