@@ -408,3 +408,42 @@ def test_surrogate_gain_over_plain_lattice(rt, lattice):
         rt.free(p)
     untrained.free()
     net.free()
+
+
+def test_entry_points_reject_bad_shapes(rt):
+    """Unsupported shapes and missing buffers are refused on the host, with a message
+    (nothing is launched)."""
+    lib = rt.lib
+    p = rt.malloc(1 << 20)
+    try:
+        rc = lib.lbm_sur_conv(ctypes.c_void_p(p), 16, ctypes.c_void_p(p), ctypes.c_void_p(p), None,
+                              ctypes.c_void_p(p), None, 32, 0, 1, 4, 64, rt.stream)
+        assert rc != 0 and b"unsupported" in lib.lbm_last_error()
+        rc = lib.lbm_sur_conv(ctypes.c_void_p(p), 64, ctypes.c_void_p(p), ctypes.c_void_p(p), None,
+                              ctypes.c_void_p(p), None, 32, 0, 1, 4, 40, rt.stream)
+        assert rc != 0 and b"multiple of 16" in lib.lbm_last_error()
+        rc = lib.lbm_sur_wgrad(ctypes.c_void_p(p), 64, ctypes.c_void_p(p), 32, 1, 4, 48, ctypes.c_void_p(p),
+                               ctypes.c_void_p(p), ctypes.c_void_p(p), rt.stream)
+        assert rc != 0 and b"multiple of 32" in lib.lbm_last_error()
+        from pylbm_amd.runtime import LbmSurNet
+        from pylbm_amd.surrogate import _f3
+        net = LbmSurNet()
+        net.hidden = 32
+        for i in range(4):
+            net.act[i] = net.wpk[i] = net.bias[i] = p
+        one = _f3((1.0, 1.0, 1.0))
+        rc = lib.lbm_sur_run(ctypes.byref(net), ctypes.c_void_p(p), one, one, one, 1, 1, 4, 16, rt.stream)
+        assert rc != 0 and b"xalt" in lib.lbm_last_error()
+        net.xalt = p + 4096
+        rc = lib.lbm_sur_run(ctypes.byref(net), ctypes.c_void_p(p), one, one, one, 1, 1, 6, 16, rt.stream)
+        assert rc != 0 and b"multiple of 4x16" in lib.lbm_last_error()
+        net.hidden = 48
+        rc = lib.lbm_sur_run(ctypes.byref(net), ctypes.c_void_p(p), one, one, one, 1, 1, 4, 16, rt.stream)
+        assert rc != 0 and b"hidden" in lib.lbm_last_error()
+        rc = lib.lbm_sur_adam(ctypes.c_void_p(p), ctypes.c_void_p(p), ctypes.c_void_p(p), ctypes.c_void_p(p),
+                              ctypes.c_void_p(p), ctypes.c_void_p(p), 4, 4, 4, 4, 1e-3, 0.9, 0.999, 1e-8, 0,
+                              rt.stream)
+        assert rc != 0 and b"step" in lib.lbm_last_error()
+        rt.sync()
+    finally:
+        rt.free(p)
