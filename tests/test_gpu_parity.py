@@ -197,3 +197,21 @@ def test_karman_full_size_fp64_parity(hip):
     for _ in range(300):
         ref.one_time_step()
     _compare(sim, ref, [cases.RHO, cases.QX, cases.QY], TOL64)
+
+
+@pytest.mark.parametrize("make", [lambda: cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0)),
+                                  lambda: cases.d1q3_advection(16, walls=True),
+                                  lambda: cases.d2q5q9_vectorial(16)])
+def test_restricted_ghost_fill_equals_full_wrap_on_device(hip, make):
+    """The default ghost handling (periodic-face slabs + copy list of the other read
+    slots) and the full wrap of every halo slot give the same run, bit for bit."""
+    d = make()
+    a, b = Simulation(d), Simulation(d)
+    b._ghost_mode = 2
+    for _ in range(30):
+        a.one_time_step()
+        b.one_time_step()
+    a.run(20)
+    b.run(20)
+    for k in range(a.ns):
+        assert np.array_equal(a.F[k], b.F[k]), k
