@@ -15,8 +15,10 @@ device sync.  Printed as one JSON line:
             counters on a plain copy kernel of known size, or null
   cpu_baseline  the oracle's C restatement of pylbm's generated Cython loop
             (FP64, AoS, -O3) on a bounded 128^3 sample, all host threads
-  configs   MLUPS of the C1 cavity (FP64), C2 channel (FP64, FP32) and C4 sphere
-            packing (FP32), each link list timed alone, C4's mass drift
+  configs   the C1 cavity (FP64, 2000 steps), C2 channel (20000 steps), C4 sphere
+            packing (5000 steps) and the C5 plain lattice run (2000 steps): MLUPS,
+            the FP32 runs' moment errors against the FP64 run on the GPU, each link
+            list timed alone, mass drifts
   surrogate the learned surrogate on the 512^2 periodic D2Q9 decaying flow:
             trained on HIP lattice trajectories, then its simulated steps per
             second against the plain FP32 lattice run and its velocity rel-L2
@@ -151,36 +153,58 @@ def _link_timing(sim, reps=20):
     return out
 
 
+def _moment_err(sim, ref, syms):
+    """Normwise max relative error per conserved moment on the scale max(||m_ref||, 1e-3 ||rho_ref||)."""
+    mref = [ref.m[x] for x in syms]
+    floor = 1e-3 * float(np.max(np.abs(mref[0])))
+    return max(float(np.max(np.abs(sim.m[x] - b))) / max(float(np.max(np.abs(b))), floor)
+               for x, b in zip(syms, mref))
+
+
 def _configs_leg():
-    """The other workloads: MLUPS of C1 (FP64), C2 (FP64 and FP32), C4 (FP32) through
-    Simulation.run, their link lists timed alone, and C4's mass drift."""
+    """The other workloads at their stated run lengths: MLUPS through Simulation.run, each
+    FP32 run's moment error against the same run in FP64 on the GPU, each link list
+    timed alone, and mass drifts where the boundaries conserve mass."""
     from pylbm_amd import Simulation, cases
 
     res = {}
-    plan = [("C1 lid-driven cavity 128^2 D2Q9 MRT FP64", lambda: cases.lid_cavity_d2q9(128), "float64", 2000),
-            ("C2 Karman channel 1024x256 D2Q9 MRT FP64", lambda: cases.karman_d2q9(), "float64", 2000),
-            ("C2 Karman channel 1024x256 D2Q9 MRT FP32", lambda: cases.karman_d2q9(), "float32", 2000),
-            ("C4 sphere packing 512x256x256 D3Q19 MRT FP32", lambda: cases.sphere_packing_d3q19(), "float32", 200)]
-    for name, make, dtype, steps in plan:
-        t0 = time.perf_counter()
-        sim = Simulation(make(), dtype=dtype)
-        setup = time.perf_counter() - t0
-        cells = int(np.prod(sim.domain.shape_in))
-        sim.run(10)
-        mass0 = sim.conserved_sums()[0] if name.startswith("C4") else None
-        ms, _ = _time_run(sim, steps)
-        entry = {"MLUPS": round(cells * steps / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms / steps, 4),
-                 "steps": steps, "cells": cells, "setup_s": round(setup, 1), "links": _link_timing(sim)}
-        if mass0 is not None:
-            mass1 = sim.conserved_sums()[0]
-            entry["mass_rel_drift"] = float("%.3e" % ((mass1 - mass0) / mass0))
+    r2, r3 = (cases.RHO, cases.QX, cases.QY), (cases.RHO, cases.QX, cases.QY, cases.QZ)
+    plan = [("C1 lid-driven cavity 128^2 D2Q9 MRT", lambda: cases.lid_cavity_d2q9(128), ("float64",), 2000, r2, False),
+            ("C2 Karman channel 1024x256 D2Q9 MRT", lambda: cases.karman_d2q9(), ("float32", "float32-shifted", "float64"),
+             20000, r2, False),
+            ("C4 sphere packing 512x256x256 D3Q19 MRT", lambda: cases.sphere_packing_d3q19(),
+             ("float32", "float32-shifted", "float64"), 5000, r3, True),
+            ("C5 decaying flow 512^2 D2Q9 MRT (plain lattice run)", lambda: cases.decaying_d2q9(512),
+             ("float32", "float64"), 2000, r2, True)]
+    for name, make, dtypes, steps, syms, mass in plan:
+        entry = {"steps": steps}
+        sims = {}
+        for dt in dtypes:
+            t0 = time.perf_counter()
+            sim = Simulation(make(), dtype=dt)
+            setup = time.perf_counter() - t0
+            cells = int(np.prod(sim.domain.shape_in))
+            m0 = sim.conserved_sums()[0] if mass else None
+            ms, _ = _time_run(sim, steps)
+            e = {"MLUPS": round(cells * steps / (ms * 1e-3) / 1e6, 1), "ms_per_step": round(ms / steps, 4),
+                 "setup_s": round(setup, 1)}
+            if mass:
+                e["mass_rel_drift"] = float("%.3e" % ((sim.conserved_sums()[0] - m0) / m0))
+            entry[dt] = e
+            entry["cells"] = cells
+            sims[dt] = sim
+        first = sims[dtypes[0]]
+        entry["links"] = _link_timing(first)
+        for dt in dtypes:
+            if dt != "float64" and "float64" in sims:
+                entry[dt]["moment_err_vs_float64"] = float("%.3e" % _moment_err(sims[dt], sims["float64"], syms))
         res[name] = entry
-        del sim
+        del sims, first
     return res
 
 
 def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), K=16, judge_every=8,
-                   tol=1e-2, horizons=(256, 2000)):
+                   tol=1e-2, horizons=(256, 2000), elem="float16"):
     """Train the surrogate on HIP lattice trajectories (512^2 periodic D2Q9, FP32, nu = 1e-3),
     then advance held-out initial states by each horizon with the plain run and with the
     judged surrogate: simulated steps per second of both (wall clock, device-resident
@@ -191,7 +215,7 @@ def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), 
 
     lat = LatticeState(n=512, dtype="float32")
     rt = lat.rt
-    net, norm, info = train_surrogate(lat, ntraj=ntraj, steps=steps, K=K, steps_per_traj=steps_per_traj)
+    net, norm, info = train_surrogate(lat, ntraj=ntraj, steps=steps, K=K, steps_per_traj=steps_per_traj, elem=elem)
     run = SurrogateRun(net, lat, norm, K=K, judge_every=judge_every, tol=tol)
     ref = rt.malloc(lat.npix * 16)
     work = rt.malloc(lat.npix * 16)
@@ -264,14 +288,15 @@ def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), 
     return {
         "workload": "512^2 periodic D2Q9 MRT decaying flow, FP32, nu=1e-3, random Fourier velocity "
                     "E(k)~k^-5/3 on |k| in [4,16], rms 0.05",
-        "network": "4x4-patch periodic 3x3 conv net 64-%d-%d-%d-64, SiLU, FP16 MFMA, FP32 accumulate"
-                   % ((info["hidden"],) * 3),
+        "network": "4x4-patch periodic 3x3 conv net 64-%d-%d-%d-64, SiLU, %s MFMA, FP32 accumulate"
+                   % ((info["hidden"],) * 3 + ({"float16": "FP16", "bfloat16": "BF16"}[elem],)),
         "K": K, "judge_every": judge_every, "tol_rel_l2": tol,
         "network_step_us": round(step_us, 2), "lattice_step_us": round(lbm_step_us, 3),
         "network_roofline": {"bound": "mfma", "achieved": round(flops / step_us * 1e-6, 1),
                              "peak": 2500.0, "unit": "TFLOP/s",
                              "frac": round(flops / step_us * 1e-6 / 2500.0, 4),
-                             "flops_per_step": flops, "dtype": "fp16 x fp16 -> fp32"},
+                             "flops_per_step": flops,
+                             "dtype": "%s x %s -> fp32" % (({"float16": "fp16", "bfloat16": "bf16"}[elem],) * 2)},
         "horizons": by_h,
         "train": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()},
     }
@@ -290,6 +315,8 @@ def main():
     ap.add_argument("--sur-traj", type=int, default=200, help="surrogate training trajectories")
     ap.add_argument("--sur-steps", type=int, default=2000, help="lattice steps per trajectory")
     ap.add_argument("--sur-train", type=int, default=100, help="optimizer steps per trajectory")
+    ap.add_argument("--sur-elem", default="float16", choices=("float16", "bfloat16"),
+                    help="16-bit MFMA operand type of the surrogate network")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="launch the plain copy kernel on a buffer of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -363,7 +390,7 @@ def main():
     if not args.no_configs:
         extra["configs"] = _configs_leg()
     if not args.no_surrogate:
-        extra["surrogate"] = _surrogate_leg(args.sur_traj, args.sur_steps, args.sur_train)
+        extra["surrogate"] = _surrogate_leg(args.sur_traj, args.sur_steps, args.sur_train, elem=args.sur_elem)
 
     out = {
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,

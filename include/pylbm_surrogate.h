@@ -13,16 +13,21 @@
  *   state        FP32 [N][Hf][Wf][4]  (rho, ux, uy, 0) on the fine (lattice) grid
  *   coarse       H x W = Hf/4 x Wf/4 pixels; channel (4 py + px) * 4 + ch of coarse
  *                pixel (Y, X) is field ch of fine pixel (4 Y + py, 4 X + px)
- *   input        FP16 [N][H][W][64]   normalised state, field 3 = 0
- *   activations  FP16 [N][H][W][C]    C = 32 or 64 hidden channels
+ *   input        E [N][H][W][64]      normalised state, field 3 = 0
+ *   activations  E [N][H][W][C]       C = 32 or 64 hidden channels
  *   output       FP32 [N][H][W][64]   increment in units of dscale (field 3 unused)
  * Hf is the mid (pylbm x) axis, Wf the inner (pylbm y) axis.  Convolutions need
  * W % 16 == 0, weight gradients W % 32 == 0.  Convolutions are periodic 3x3;
  * taps t = 3 (dy + 1) + (dx + 1) read pixel (y + dy, x + dx).  Master weights:
- * FP32 w[9][cin][cout], b[cout].  Packed operands (FP16, written by lbm_sur_pack):
+ * FP32 w[9][cin][cout], b[cout].  Packed operands (E, written by lbm_sur_pack):
  *   forward  A [cout][9 cin],  k = tap * cin + ci
  *   backward A [cin][9 cout],  the input-gradient convolution: weights of tap
  *            8 - t with channel roles swapped, k = tap * cout + co
+ * E is the 16-bit element type of every input, activation, gradient and packed
+ * operand buffer, chosen per call by `elem`: LBM_SUR_F16 (IEEE half) or
+ * LBM_SUR_BF16 (bfloat16); the MFMA products are v_mfma_f32_16x16x32_f16 / _bf16
+ * with FP32 accumulation, and FP32 -> E conversions round to nearest even.  All
+ * buffers of one network must use the same E.
  * All calls are asynchronous on `stream` unless they return a host value.
  */
 #ifndef PYLBM_SURROGATE_H
@@ -36,40 +41,43 @@
 extern "C" {
 #endif
 
+#define LBM_SUR_F16 0  /* elem: IEEE binary16 */
+#define LBM_SUR_BF16 1 /* elem: bfloat16 */
+
 /*
  * One periodic 3x3 convolution on the MFMA units (FP32 accumulate).
- * epilogue 0: z = conv + b, out = silu(z) (FP16, coutp channels), d_zout = z (FP16) if not NULL
+ * epilogue 0: z = conv + b, out = silu(z) (E, coutp channels), d_zout = z (E) if not NULL
  * epilogue 1: out = conv + b (FP32, coutp channels)
- * epilogue 2: out = conv * silu'(d_zin) (FP16): the input gradient of a layer
+ * epilogue 2: out = conv * silu'(d_zin) (E): the input gradient of a layer
  *             followed by the SiLU of the previous one (d_wpk = backward operand,
  *             d_bias unused)
  * Supported (cinp, coutp, epilogue): (64,64,0) (64,32,0) (32,32,0) (32,64,1) (64,64,1)
  * (64,64,2) (64,32,2) (32,32,2).
  */
 int lbm_sur_conv(const void* d_in, int cinp, const void* d_wpk, const float* d_bias, const void* d_zin,
-                 void* d_out, void* d_zout, int coutp, int epilogue, int N, int H, int W, void* stream);
+                 void* d_out, void* d_zout, int coutp, int epilogue, int N, int H, int W, int elem, void* stream);
 
 /*
  * The last (64-channel output) convolution of an inference step fused with its use:
  * d_state += (conv + b) * dscale for the 3 fields of every fine pixel, and the next
- * network input (d_state - shift) / scale written to d_xnext (FP16, coarse layout;
+ * network input (d_state - shift) / scale written to d_xnext (E, coarse layout;
  * must not alias d_in).  dscale, shift, scale: 3 host floats.  cinp = 32 or 64.
  */
 int lbm_sur_apply_step(const void* d_in, int cinp, const void* d_wpk, const float* d_bias, float* d_state,
                        void* d_xnext, const float* dscale, const float* shift, const float* scale, int N, int H,
-                       int W, void* stream);
+                       int W, int elem, void* stream);
 
-/* the buffers of one network for lbm_sur_run: act[0] = input (FP16, 64 channels),
- * act[1..3] = hidden activations (FP16, `hidden` channels; not used by
+/* the buffers of one network for lbm_sur_run: act[0] = input (E, 64 channels),
+ * act[1..3] = hidden activations (E, `hidden` channels; not used by
  * lbm_sur_run), xalt = a second input buffer of act[0]'s size, wpk / bias =
- * forward operands and biases of the 4 layers */
+ * forward operands and biases of the 4 layers, elem = E (LBM_SUR_F16 / _BF16) */
 typedef struct lbm_sur_net {
   void* act[4];
   void* xalt;
   const void* wpk[4];
   const float* bias[4];
   int32_t hidden;
-  int32_t _pad;
+  int32_t elem;
 } lbm_sur_net;
 
 /* nsteps inference steps without returning to the host, one launch per step: a
@@ -86,28 +94,28 @@ int lbm_sur_wgrad_blocks(void);
 
 /*
  * Weight and bias gradients of one convolution: d_dwt [9][cdz][cinp] = sum over
- * pixels of dZ[p][co] X[p + d_tap][ci], d_db[cdz] = sum dZ.  d_x: FP16 input (cinp
- * channels), d_dz: FP16 gradient of the pre-activation (cdz channels).  d_partial:
+ * pixels of dZ[p][co] X[p + d_tap][ci], d_db[cdz] = sum dZ.  d_x: E input (cinp
+ * channels), d_dz: E gradient of the pre-activation (cdz channels).  d_partial:
  * lbm_sur_wgrad_blocks() * (9 cdz cinp + cdz) floats; the partial sums are reduced
  * in a fixed order (results are deterministic).  cinp, cdz in {32, 64}.
  */
 int lbm_sur_wgrad(const void* d_x, int cinp, const void* d_dz, int cdz, int N, int H, int W, float* d_partial,
-                  float* d_dwt, float* d_db, void* stream);
+                  float* d_dwt, float* d_db, int elem, void* stream);
 
 /*
  * Mean-square loss of the FP32 output against the FP32 target over the 48 value
  * channels (field 3 excluded) of npix coarse pixels; d_dy = 2 gscale (y - t) as
- * FP16 [npix][64]: gscale = 1 / (48 npix) is the exact loss gradient, gscale = 1
- * the loss-scaled one the FP16 backward pass uses (Adam with eps * 48 npix then
+ * E [npix][64]: gscale = 1 / (48 npix) is the exact loss gradient, gscale = 1
+ * the loss-scaled one the 16-bit backward pass uses (Adam with eps * 48 npix then
  * takes the identical step).  d_partial: 256 doubles.  h_loss (host, may be NULL:
  * then nothing synchronises) = the loss.
  */
 int lbm_sur_mse(const float* d_y, const float* d_t, int64_t npix, float gscale, void* d_dy, double* d_partial,
-                double* h_loss, void* stream);
+                double* h_loss, int elem, void* stream);
 
 /* pack the FP32 master weights of one layer into the forward (and, if d_abwd is
- * not NULL, backward) FP16 MFMA operands; cin <= cinp <= 64, cout <= coutp <= 64 */
-int lbm_sur_pack(const float* d_w, int cin, int cout, int cinp, int coutp, void* d_afwd, void* d_abwd,
+ * not NULL, backward) E MFMA operands; cin <= cinp <= 64, cout <= coutp <= 64 */
+int lbm_sur_pack(const float* d_w, int cin, int cout, int cinp, int coutp, void* d_afwd, void* d_abwd, int elem,
                  void* stream);
 
 /* one Adam step (bias-corrected, step >= 1) on w[9][cin][cout] and b[cout]; d_m / d_v
@@ -116,10 +124,10 @@ int lbm_sur_pack(const float* d_w, int cin, int cout, int cinp, int coutp, void*
 int lbm_sur_adam(float* d_w, float* d_b, float* d_m, float* d_v, const float* d_dwt, const float* d_db, int cin,
                  int cout, int cinp, int coutp, float lr, float b1, float b2, float eps, int step, void* stream);
 
-/* network input (FP16 coarse [N][Hf/4][Wf/4][64]) = (state - shift) / scale per field
+/* network input (E coarse [N][Hf/4][Wf/4][64]) = (state - shift) / scale per field
  * (3 host floats each) */
 int lbm_sur_make_input(const float* d_state, int N, int Hf, int Wf, const float* shift, const float* scale,
-                       void* d_xin, void* stream);
+                       void* d_xin, int elem, void* stream);
 /* training target (FP32 coarse [N][Hf/4][Wf/4][64]) = (s1 - s0) / dscale */
 int lbm_sur_make_target(const float* d_s0, const float* d_s1, int N, int Hf, int Wf, const float* dscale,
                         float* d_t, void* stream);

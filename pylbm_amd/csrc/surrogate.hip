@@ -6,18 +6,20 @@
 // between layers, C = 32 or 64) returns the increment of every fine value over
 // K lattice steps in the same layout.
 //
-// Activations, gradients and packed weights are FP16 NHWC (channels contiguous);
+// Activations, gradients and packed weights are 16-bit NHWC (channels contiguous),
+// FP16 or BF16 by the caller's choice (every kernel is instantiated for both);
 // gradients are loss-scaled (dY = 2 (y - t), no 1 / count).  Every convolution is
 // an implicit GEMM on the MFMA units with FP32 accumulation:
 //   forward / input-gradient:  D[co][pixel] = A[co][tap * CINP + ci] * B[tap * CINP + ci][pixel]
 //     A = packed weights, B = the 3x3 neighbourhood gathered from NHWC with
-//     v_mfma_f32_16x16x32_f16, 32 input channels of one tap per K step.  A wave
+//     v_mfma_f32_16x16x32_f16 / _bf16, 32 input channels of one tap per K step.  A wave
 //     computes 16 consecutive pixels of one row for MT * 16 output channels.
 //   weight gradient: dW[tap]^T[co][ci] = sum_p dZ[p][co] X[p + d_tap][ci], K = pixels,
 //     operands staged transposed in LDS (aligned 16-byte reads), one tap per
 //     workgroup row of the grid, per-workgroup partials reduced in a second pass
 //     in a fixed order (deterministic).
-// Lane layouts (checked on the device against a CPU product): for 16x16x32, lane l
+// Lane layouts (checked on the device against a CPU product; the same for both
+// element types): for 16x16x32, lane l
 // holds A row l % 16 and B column l % 16 with k = (l / 16) * 8 + [0, 8), and D
 // column l % 16, rows (l / 16) * 4 + [0, 4).
 #include "lbm_device.h"
@@ -25,14 +27,35 @@
 
 namespace {
 
-typedef _Float16 h16;
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
-constexpr int EPI_ACT = 0;    // z = acc + b, a = silu(z): store a (and z) as FP16
+// The 16-bit element type E of activations, gradients and packed weights: its 4- and
+// 8-wide vectors and the 16x16x32 MFMA that multiplies two 8-wide operands.
+// Conversions from float ((E)x) round to nearest even for both types.
+template <typename E>
+struct Elem;
+template <>
+struct Elem<_Float16> {
+  typedef _Float16 v4 __attribute__((ext_vector_type(4)));
+  typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+  static __device__ __forceinline__ float4v mfma(v8 a, v8 b, float4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Elem<__bf16> {
+  typedef __bf16 v4 __attribute__((ext_vector_type(4)));
+  typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+  static __device__ __forceinline__ float4v mfma(v8 a, v8 b, float4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+static_assert(sizeof(_Float16) == 2 && sizeof(__bf16) == 2, "16-bit elements");
+constexpr int EB = 2;  // bytes per element
+
+constexpr int EPI_ACT = 0;    // z = acc + b, a = silu(z): store a (and z) as E
 constexpr int EPI_LIN = 1;    // y = acc + b as FP32
-constexpr int EPI_BWD = 2;    // g = acc * silu'(z_in): store FP16
+constexpr int EPI_BWD = 2;    // g = acc * silu'(z_in): store E
 constexpr int EPI_APPLY = 3;  // fine state += (acc + b) * dscale, next network input
 
 constexpr int WG_PIX = 32;             // pixels per weight-gradient block iteration
@@ -52,7 +75,7 @@ __device__ __forceinline__ void staged_copy(Ld ld, St st) {
 struct ApplyArgs {
   float4v dscale, shift, inv_scale;
   float* state;  // fine FP32 [N][4H][4W][4]
-  h16* xnext;    // next network input, coarse FP16 [N][H][W][64]
+  void* xnext;   // next network input, coarse E [N][H][W][64]
 };
 
 // ---------------------------------------------------------------- conv 3x3
@@ -62,12 +85,14 @@ struct ApplyArgs {
 // both are padded by 16 bytes per pixel / row so that the 16 rows (pixels) one
 // MFMA operand read touches fall in distinct banks.  Then each wave runs its
 // 9 * CINP / 32 K steps of MT = COUTP / 16 independent MFMA chains from LDS.
-template <int CINP, int COUTP, int EPI>
-__global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in, const h16* __restrict__ wpk,
+template <typename E, int CINP, int COUTP, int EPI>
+__global__ void __launch_bounds__(256) conv3x3_kernel(const E* __restrict__ in, const E* __restrict__ wpk,
                                                       const float* __restrict__ bias,
-                                                      const h16* __restrict__ zin, void* __restrict__ out,
-                                                      h16* __restrict__ zout, int N, int H, int W,
+                                                      const E* __restrict__ zin, void* __restrict__ out,
+                                                      E* __restrict__ zout, int N, int H, int W,
                                                       ApplyArgs ap) {
+  typedef typename Elem<E>::v4 half4;
+  typedef typename Elem<E>::v8 half8;
   constexpr int MT = COUTP / 16;
   constexpr int KP = 9 * CINP;
   constexpr int KPP = KP + 8;
@@ -75,9 +100,10 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in
   constexpr int CIP = CINP + 8;
   constexpr int SEG = 64, SPAN = SEG + 2;
   static_assert(CINP % 32 == 0 && COUTP % 16 == 0, "channel tiling");
-  extern __shared__ __attribute__((aligned(16))) h16 smem[];
-  h16* sw = smem;                       // [COUTP][KPP]
-  h16* sx = smem + COUTP * KPP;         // [3][SPAN][CIP]
+  extern __shared__ uint4 smem_u4[];
+  E* smem = reinterpret_cast<E*>(smem_u4);
+  E* sw = smem;                         // [COUTP][KPP]
+  E* sx = smem + COUTP * KPP;           // [3][SPAN][CIP]
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int segs = (W + SEG - 1) / SEG;
   const int n = blockIdx.x / (H * segs);
@@ -88,7 +114,7 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in
   {
     constexpr int NW8 = COUTP * (KP / 8);
     constexpr int NX8 = 3 * SPAN * (CINP / 8);
-    const h16* img = in + (int64_t)n * H * W * CINP;
+    const E* img = in + (int64_t)n * H * W * CINP;
     staged_copy<NW8>(
         [&](int e) {
           half8 v;
@@ -118,7 +144,7 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in
   for (int mt = 0; mt < MT; ++mt) acc[mt] = float4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
-    const h16* bx = sx + ((t / 3) * SPAN + wv * 16 + r + t % 3) * CIP + g * 8;
+    const E* bx = sx + ((t / 3) * SPAN + wv * 16 + r + t % 3) * CIP + g * 8;
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
       half8 b;
@@ -127,7 +153,7 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in
       for (int mt = 0; mt < MT; ++mt) {
         half8 a;
         __builtin_memcpy(&a, sw + (mt * 16 + r) * KPP + t * CINP + kc * 32 + g * 8, 16);
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[mt], 0, 0, 0);
+        acc[mt] = Elem<E>::mfma(a, b, acc[mt]);
       }
     }
   }
@@ -146,10 +172,10 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const float z = acc[mt][q] + bias[c0 + q];
-        zb[q] = (h16)z;
-        ab[q] = (h16)(z * sigmoid(z));
+        zb[q] = (E)z;
+        ab[q] = (E)(z * sigmoid(z));
       }
-      __builtin_memcpy((h16*)out + p * COUTP + c0, &ab, 8);
+      __builtin_memcpy((E*)out + p * COUTP + c0, &ab, 8);
       if (zout) __builtin_memcpy(zout + p * COUTP + c0, &zb, 8);
     } else if constexpr (EPI == EPI_BWD) {
       half4 zb, gb;
@@ -158,9 +184,9 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in
       for (int q = 0; q < 4; ++q) {
         const float z = (float)zb[q];
         const float s = sigmoid(z);
-        gb[q] = (h16)(acc[mt][q] * (s * (1.f + z * (1.f - s))));
+        gb[q] = (E)(acc[mt][q] * (s * (1.f + z * (1.f - s))));
       }
-      __builtin_memcpy((h16*)out + p * COUTP + c0, &gb, 8);
+      __builtin_memcpy((E*)out + p * COUTP + c0, &gb, 8);
     } else {
       static_assert(EPI != EPI_APPLY || COUTP == 64, "apply epilogue needs the 64-channel patch layout");
       // channels c0..c0+3 are the 4 fields of fine pixel (4 y + py, 4 x + px)
@@ -173,11 +199,11 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const h16* __restrict__ in
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         s[q] += (acc[mt][q] + bias[c0 + q]) * ap.dscale[q];
-        xb[q] = (h16)((s[q] - ap.shift[q]) * ap.inv_scale[q]);
+        xb[q] = (E)((s[q] - ap.shift[q]) * ap.inv_scale[q]);
       }
-      xb[3] = (h16)0.f;
+      xb[3] = (E)0.f;
       __builtin_memcpy(ap.state + fp * 4, &s, 16);
-      __builtin_memcpy(ap.xnext + p * 64 + c0, &xb, 8);
+      __builtin_memcpy((E*)ap.xnext + p * 64 + c0, &xb, 8);
     }
   }
 }
@@ -209,8 +235,8 @@ struct FusedGeom {
   // two weight buffers (layers 0, 2 | 1, 3) when they fit: the next layer's weights
   // arrive while the current layer computes
   static constexpr int WB0 = W0 > WH ? W0 : WH, WB1 = WH > W3 ? WH : W3;
-  static constexpr bool DB = (size_t)(BUF_A + BUF_B + WB0 + WB1) * sizeof(h16) <= 160 * 1024;
-  static constexpr size_t LDS = (size_t)(BUF_A + BUF_B + (DB ? WB0 + WB1 : WBUF)) * sizeof(h16);
+  static constexpr bool DB = (size_t)(BUF_A + BUF_B + WB0 + WB1) * EB <= 160 * 1024;
+  static constexpr size_t LDS = (size_t)(BUF_A + BUF_B + (DB ? WB0 + WB1 : WBUF)) * EB;
 };
 
 template <int CIN, int COUT>
@@ -220,11 +246,11 @@ struct LayerW {
 };
 
 // Asynchronous copy into LDS (gfx950 global_load_lds_dwordx4): NU 16-byte units,
-// unit u lands at dst + 8 u halves and is read from src_of(u).  A wave issues the
+// unit u lands at dst + 8 u elements and is read from src_of(u).  A wave issues the
 // units [64 k, 64 k + 64) for k = wave, wave + 4, ...; the caller waits
 // (s_waitcnt + barrier) before reading dst.
-template <int NU, typename F>
-__device__ __forceinline__ void dma_units(h16* dst, F src_of) {
+template <int NU, typename E, typename F>
+__device__ __forceinline__ void dma_units(E* dst, F src_of) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll 4
   for (int k = wv; k * 64 < NU; k += 4) {
@@ -235,10 +261,10 @@ __device__ __forceinline__ void dma_units(h16* dst, F src_of) {
   }
 }
 
-// weights of one layer, global [COUT][9 CIN] -> LDS rows padded to 9 CIN + 8 halves
+// weights of one layer, global [COUT][9 CIN] -> LDS rows padded to 9 CIN + 8 elements
 // (the last unit of a row is padding: it receives a copy of the row's last unit)
-template <int CIN, int COUT>
-__device__ __forceinline__ void dma_weights(const h16* __restrict__ w, h16* sw) {
+template <int CIN, int COUT, typename E>
+__device__ __forceinline__ void dma_weights(const E* __restrict__ w, E* sw) {
   constexpr int UR = 9 * CIN / 8 + 1;
   dma_units<COUT * UR>(sw, [&](int u) {
     const int row = u / UR, c = u % UR;
@@ -250,10 +276,12 @@ __device__ __forceinline__ void dma_wait() { __builtin_amdgcn_s_waitcnt(0); }
 
 // Layer LAYER of the tile: input region (RO + 2) x (WO + 2) at `src` (stride CIPI),
 // output region RO x WO at `dst` (stride CIPO), or the state / next input for LAYER 3.
-template <int C, int LAYER, int CIN, int COUT>
-__device__ __forceinline__ void fused_layer(const h16* __restrict__ src, h16* __restrict__ dst,
-                                            const h16* __restrict__ sw, const float* __restrict__ bias,
+template <typename E, int C, int LAYER, int CIN, int COUT>
+__device__ __forceinline__ void fused_layer(const E* __restrict__ src, E* __restrict__ dst,
+                                            const E* __restrict__ sw, const float* __restrict__ bias,
                                             int n, int y0, int x0, int H, int W, const ApplyArgs& ap) {
+  typedef typename Elem<E>::v4 half4;
+  typedef typename Elem<E>::v8 half8;
   constexpr int RO = FT_Y + 2 * (3 - LAYER), WO = FT_X + 2 * (3 - LAYER);
   constexpr int WI = WO + 2;
   constexpr int CIPI = CIN + 8, CIPO = COUT + 8;
@@ -287,9 +315,9 @@ __device__ __forceinline__ void fused_layer(const h16* __restrict__ src, h16* __
     // K steps ks = t * KC + kc, software-pipelined: the operands of step ks + 1 are
     // read from LDS while the MFMAs of step ks issue
     constexpr int KC = CIN / 32, NK = 9 * KC;
-    const h16* bbase0 = src + (prs[0] * WI + pqs[0]) * CIPI + g * 8;
-    const h16* bbase1 = src + (prs[1] * WI + pqs[1]) * CIPI + g * 8;
-    const h16* abase = sw + r * KPP + g * 8;
+    const E* bbase0 = src + (prs[0] * WI + pqs[0]) * CIPI + g * 8;
+    const E* bbase1 = src + (prs[1] * WI + pqs[1]) * CIPI + g * 8;
+    const E* abase = sw + r * KPP + g * 8;
     // ring of FUSED_PREFETCH + 1 operand sets
     constexpr int D = FUSED_PREFETCH;
     half8 bop[D + 1][2], aop[D + 1][MT];
@@ -309,8 +337,8 @@ __device__ __forceinline__ void fused_layer(const h16* __restrict__ src, h16* __
       const int c = ks % (D + 1);
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        acc[0][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aop[c][mt], bop[c][0], acc[0][mt], 0, 0, 0);
-        if constexpr (TP == 2) acc[1][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aop[c][mt], bop[c][1], acc[1][mt], 0, 0, 0);
+        acc[0][mt] = Elem<E>::mfma(aop[c][mt], bop[c][0], acc[0][mt]);
+        if constexpr (TP == 2) acc[1][mt] = Elem<E>::mfma(aop[c][mt], bop[c][1], acc[1][mt]);
       }
     }
 #pragma unroll
@@ -325,7 +353,7 @@ __device__ __forceinline__ void fused_layer(const h16* __restrict__ src, h16* __
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const float z = acc[u][mt][q] + bias[c0 + q];
-            ab[q] = (h16)(z * sigmoid(z));
+            ab[q] = (E)(z * sigmoid(z));
           }
           __builtin_memcpy(dst + pc * CIPO + c0, &ab, 8);
         } else {
@@ -339,37 +367,38 @@ __device__ __forceinline__ void fused_layer(const h16* __restrict__ src, h16* __
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             sv[q] += (acc[u][mt][q] + bias[c0 + q]) * ap.dscale[q];
-            xb[q] = (h16)((sv[q] - ap.shift[q]) * ap.inv_scale[q]);
+            xb[q] = (E)((sv[q] - ap.shift[q]) * ap.inv_scale[q]);
           }
-          xb[3] = (h16)0.f;
+          xb[3] = (E)0.f;
           __builtin_memcpy(ap.state + fp * 4, &sv, 16);
-          __builtin_memcpy(ap.xnext + (((int64_t)n * H + y) * W + x) * 64 + c0, &xb, 8);
+          __builtin_memcpy((E*)ap.xnext + (((int64_t)n * H + y) * W + x) * 64 + c0, &xb, 8);
         }
       }
     }
   }
 }
 
-template <int C>
-__global__ void __launch_bounds__(256) fused_step_kernel(const h16* __restrict__ xin, const h16* __restrict__ w0,
-                                                         const h16* __restrict__ w1, const h16* __restrict__ w2,
-                                                         const h16* __restrict__ w3, const float* __restrict__ b0,
+template <typename E, int C>
+__global__ void __launch_bounds__(256) fused_step_kernel(const E* __restrict__ xin, const E* __restrict__ w0,
+                                                         const E* __restrict__ w1, const E* __restrict__ w2,
+                                                         const E* __restrict__ w3, const float* __restrict__ b0,
                                                          const float* __restrict__ b1, const float* __restrict__ b2,
                                                          const float* __restrict__ b3, int N, int H, int W,
                                                          ApplyArgs ap) {
   using G = FusedGeom<C>;
-  extern __shared__ __attribute__((aligned(16))) h16 smem[];
-  h16* bufA = smem;
-  h16* bufB = smem + G::BUF_A;
-  h16* sw0 = smem + G::BUF_A + G::BUF_B;
-  h16* sw1 = G::DB ? sw0 + G::WB0 : sw0;
+  extern __shared__ uint4 smem_u4[];
+  E* smem = reinterpret_cast<E*>(smem_u4);
+  E* bufA = smem;
+  E* bufB = smem + G::BUF_A;
+  E* sw0 = smem + G::BUF_A + G::BUF_B;
+  E* sw1 = G::DB ? sw0 + G::WB0 : sw0;
   const int tx = W / FT_X, ty = H / FT_Y;
   const int n = blockIdx.x / (tx * ty);
   const int y0 = ((blockIdx.x / tx) % ty) * FT_Y, x0 = (blockIdx.x % tx) * FT_X;
   // layer-0 input region: rows y0 - 4 .. y0 + FT_Y + 3, columns x0 - 4 .. x0 + FT_X + 3
   // (periodic), 9 units per pixel (8 of channels + 1 of padding)
   {
-    const h16* img = xin + (int64_t)n * H * W * 64;
+    const E* img = xin + (int64_t)n * H * W * 64;
     dma_units<G::R_IN * G::W_IN * 9>(bufA, [&](int u) {
       const int q = u / 9, c = u % 9;
       const int rr = q / G::W_IN, cc = q % G::W_IN;
@@ -381,7 +410,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const h16* __restrict__
   if constexpr (G::DB) dma_weights<C, C>(w1, sw1);
   dma_wait();
   __syncthreads();
-  fused_layer<C, 0, 64, C>(bufA, bufB, sw0, b0, n, y0, x0, H, W, ap);
+  fused_layer<E, C, 0, 64, C>(bufA, bufB, sw0, b0, n, y0, x0, H, W, ap);
   __syncthreads();
   if constexpr (G::DB) {
     dma_weights<C, C>(w2, sw0);
@@ -390,7 +419,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const h16* __restrict__
     dma_wait();
     __syncthreads();
   }
-  fused_layer<C, 1, C, C>(bufB, bufA, sw1, b1, n, y0, x0, H, W, ap);
+  fused_layer<E, C, 1, C, C>(bufB, bufA, sw1, b1, n, y0, x0, H, W, ap);
   dma_wait();
   __syncthreads();
   if constexpr (G::DB) {
@@ -400,7 +429,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const h16* __restrict__
     dma_wait();
     __syncthreads();
   }
-  fused_layer<C, 2, C, C>(bufA, bufB, sw0, b2, n, y0, x0, H, W, ap);
+  fused_layer<E, C, 2, C, C>(bufA, bufB, sw0, b2, n, y0, x0, H, W, ap);
   dma_wait();
   __syncthreads();
   if constexpr (!G::DB) {
@@ -408,24 +437,26 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const h16* __restrict__
     dma_wait();
     __syncthreads();
   }
-  fused_layer<C, 3, C, 64>(bufB, nullptr, sw1, b3, n, y0, x0, H, W, ap);
+  fused_layer<E, C, 3, C, 64>(bufB, nullptr, sw1, b3, n, y0, x0, H, W, ap);
 }
 
 // -------------------------------------------------------- weight gradient
 // Workgroup (bx, tap): partial sums of dW[tap]^T[co][ci] = sum_p dZ[p][co] X[p + d_tap][ci]
 // over the blocks of WG_PIX pixels bx, bx + gridDim.x, ...; the tap-4 (centre) row of
 // the grid also sums db[co] = sum_p dZ[p][co].
-template <int CINP, int CDZ>
-__global__ void __launch_bounds__(256) conv3x3_wgrad_kernel(const h16* __restrict__ x, const h16* __restrict__ dz,
+template <typename E, int CINP, int CDZ>
+__global__ void __launch_bounds__(256) conv3x3_wgrad_kernel(const E* __restrict__ x, const E* __restrict__ dz,
                                                             float* __restrict__ partial,
                                                             float* __restrict__ partial_b, int N, int H, int W) {
+  typedef typename Elem<E>::v4 half4;
+  typedef typename Elem<E>::v8 half8;
   constexpr int MT = CDZ / 16, NT = CINP / 16;
   constexpr int TILES = MT * NT;
   static_assert(TILES % 4 == 0, "tiles per wave");
   constexpr int PER_WAVE = TILES / 4;
   constexpr int LP = WG_PIX + 8;  // padded pixel stride of the transposed tiles (80 B)
-  __shared__ __attribute__((aligned(16))) h16 sdz[CDZ][LP];
-  __shared__ __attribute__((aligned(16))) h16 sx[CINP][LP];
+  __shared__ __attribute__((aligned(16))) E sdz[CDZ][LP];
+  __shared__ __attribute__((aligned(16))) E sx[CINP][LP];
   const int tap = blockIdx.y;
   const int dy = tap / 3 - 1, dx = tap % 3 - 1;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -467,7 +498,7 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_kernel(const h16* __restric
       half8 a, b;
       __builtin_memcpy(&a, &sdz[mt * 16 + r][g * 8], 16);
       __builtin_memcpy(&b, &sx[nt * 16 + r][g * 8], 16);
-      acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[q], 0, 0, 0);
+      acc[q] = Elem<E>::mfma(a, b, acc[q]);
     }
   }
   // partial[bx][tap][co][ci]
@@ -503,10 +534,12 @@ __global__ void __launch_bounds__(256) sum_partials_kernel(const float* __restri
   out[i] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
-// Masked mean-square loss over the 48 value channels of 64; dY = 2 gscale (y - t) as FP16
+// Masked mean-square loss over the 48 value channels of 64; dY = 2 gscale (y - t) as E
+template <typename E>
 __global__ void __launch_bounds__(256) mse_kernel(const float* __restrict__ y, const float* __restrict__ tg,
-                                                  int64_t nq, float gscale, h16* __restrict__ dy,
+                                                  int64_t nq, float gscale, E* __restrict__ dy,
                                                   double* __restrict__ partial) {
+  typedef typename Elem<E>::v4 half4;
   __shared__ double sh[256];
   double s = 0.0;
   // one float4 = the 4 fields of one fine pixel; field 3 carries nothing
@@ -519,7 +552,7 @@ __global__ void __launch_bounds__(256) mse_kernel(const float* __restrict__ y, c
     for (int q = 0; q < 4; ++q) {
       const float d = q < 3 ? a[q] - b[q] : 0.f;
       s += (double)d * d;
-      gb[q] = (h16)(2.f * gscale * d);
+      gb[q] = (E)(2.f * gscale * d);
     }
     __builtin_memcpy(dy + i * 4, &gb, 8);
   }
@@ -534,20 +567,21 @@ __global__ void __launch_bounds__(256) mse_kernel(const float* __restrict__ y, c
 
 // master weights W[tap][ci][co] (FP32) -> forward A operand [coutp][9 cinp] and the
 // input-gradient A operand [cinp][9 coutp] (taps flipped, channel roles swapped)
+template <typename E>
 __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ w, int cin, int cout, int cinp,
-                                                   int coutp, h16* __restrict__ afwd, h16* __restrict__ abwd) {
+                                                   int coutp, E* __restrict__ afwd, E* __restrict__ abwd) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = coutp * 9 * cinp;
   if (i >= n) return;
   {
     const int co = i / (9 * cinp), k = i % (9 * cinp), tap = k / cinp, ci = k % cinp;
     const float v = (co < cout && ci < cin) ? w[(tap * cin + ci) * cout + co] : 0.f;
-    afwd[i] = (h16)v;
+    afwd[i] = (E)v;
   }
   if (abwd) {
     const int ci = i / (9 * coutp), k = i % (9 * coutp), tap = k / coutp, co = k % coutp;
     const float v = (ci < cin && co < cout) ? w[((8 - tap) * cin + ci) * cout + co] : 0.f;
-    abwd[i] = (h16)v;
+    abwd[i] = (E)v;
   }
 }
 
@@ -577,10 +611,12 @@ __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ w, float*
 }
 
 // ------------------------------------------------------- state <-> network
-// fine state FP32 [N][Hf][Wf][4] -> coarse input FP16 [N][Hf/4][Wf/4][64] (normalised);
+// fine state FP32 [N][Hf][Wf][4] -> coarse input E [N][Hf/4][Wf/4][64] (normalised);
 // one thread per fine pixel
+template <typename E>
 __global__ void __launch_bounds__(256) make_input_kernel(const float* __restrict__ st, int N, int Hf, int Wf,
-                                                         float4v shift, float4v inv_scale, h16* __restrict__ xin) {
+                                                         float4v shift, float4v inv_scale, E* __restrict__ xin) {
+  typedef typename Elem<E>::v4 half4;
   const int64_t fp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (fp >= (int64_t)N * Hf * Wf) return;
   const int fx = (int)(fp % Wf), fy = (int)((fp / Wf) % Hf), n = (int)(fp / ((int64_t)Wf * Hf));
@@ -588,7 +624,7 @@ __global__ void __launch_bounds__(256) make_input_kernel(const float* __restrict
   __builtin_memcpy(&s, st + fp * 4, 16);
   half4 o;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) o[q] = (h16)(q < 3 ? (s[q] - shift[q]) * inv_scale[q] : 0.f);
+  for (int q = 0; q < 4; ++q) o[q] = (E)(q < 3 ? (s[q] - shift[q]) * inv_scale[q] : 0.f);
   const int H = Hf / 4, W = Wf / 4;
   const int64_t cp = ((int64_t)n * H + fy / 4) * W + fx / 4;
   __builtin_memcpy(xin + cp * 64 + ((fy & 3) * 4 + (fx & 3)) * 4, &o, 8);
@@ -740,37 +776,50 @@ __global__ void __launch_bounds__(256) populations_to_state_kernel(const float* 
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-template <int CINP, int COUTP, int EPI>
-int conv_launch(const h16* in, const h16* wpk, const float* bias, const h16* zin, void* out, h16* zout, int N,
+// the element type of an entry point's 16-bit buffers: f(Tag<E>()) for E = _Float16 / __bf16
+template <typename E>
+struct Tag {
+  typedef E type;
+};
+
+template <typename F>
+int with_elem(int elem, const char* what, F&& f) {
+  if (elem == LBM_SUR_F16) return f(Tag<_Float16>());
+  if (elem == LBM_SUR_BF16) return f(Tag<__bf16>());
+  LBM_REQUIRE(false, "%s: element type %d is neither LBM_SUR_F16 (0) nor LBM_SUR_BF16 (1)", what, elem);
+}
+
+template <typename E, int CINP, int COUTP, int EPI>
+int conv_launch(const void* in, const void* wpk, const float* bias, const void* zin, void* out, void* zout, int N,
                 int H, int W, const ApplyArgs& ap, hipStream_t s) {
-  constexpr size_t lds = ((size_t)COUTP * (9 * CINP + 8) + 3 * 66 * (CINP + 8)) * sizeof(h16);
+  constexpr size_t lds = ((size_t)COUTP * (9 * CINP + 8) + 3 * 66 * (CINP + 8)) * EB;
   static_assert(lds <= 160 * 1024, "LDS budget");
   static bool attr = false;  // allow more than 64 KiB of dynamic LDS once per instantiation
   if (!attr) {
-    LBM_CHECK_HIP(hipFuncSetAttribute((const void*)conv3x3_kernel<CINP, COUTP, EPI>,
+    LBM_CHECK_HIP(hipFuncSetAttribute((const void*)conv3x3_kernel<E, CINP, COUTP, EPI>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr = true;
   }
   const int64_t groups = (int64_t)N * H * ((W + 63) / 64);
-  conv3x3_kernel<CINP, COUTP, EPI><<<(unsigned)groups, 256, lds, s>>>(in, wpk, bias, zin, out, zout, N, H, W, ap);
+  conv3x3_kernel<E, CINP, COUTP, EPI><<<(unsigned)groups, 256, lds, s>>>(
+      (const E*)in, (const E*)wpk, bias, (const E*)zin, out, (E*)zout, N, H, W, ap);
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
 }
 
-template <int C>
-int fused_launch(const lbm_sur_net* net, const h16* xin, h16* xout, const ApplyArgs& ap, int N, int H, int W,
-                        hipStream_t s) {
+template <typename E, int C>
+int fused_launch(const lbm_sur_net* net, const void* xin, const ApplyArgs& ap, int N, int H, int W, hipStream_t s) {
   using G = FusedGeom<C>;
   static_assert(G::LDS <= 160 * 1024, "LDS budget");
   static bool attr = false;
   if (!attr) {
-    LBM_CHECK_HIP(hipFuncSetAttribute((const void*)fused_step_kernel<C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)G::LDS));
+    LBM_CHECK_HIP(hipFuncSetAttribute((const void*)fused_step_kernel<E, C>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)G::LDS));
     attr = true;
   }
   const unsigned blocks = (unsigned)((int64_t)N * (H / FT_Y) * (W / FT_X));
-  fused_step_kernel<C><<<blocks, 256, G::LDS, s>>>(
-      xin, (const h16*)net->wpk[0], (const h16*)net->wpk[1], (const h16*)net->wpk[2], (const h16*)net->wpk[3],
+  fused_step_kernel<E, C><<<blocks, 256, G::LDS, s>>>(
+      (const E*)xin, (const E*)net->wpk[0], (const E*)net->wpk[1], (const E*)net->wpk[2], (const E*)net->wpk[3],
       net->bias[0], net->bias[1], net->bias[2], net->bias[3], N, H, W, ap);
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
@@ -783,7 +832,7 @@ using namespace lbm;
 extern "C" {
 
 int lbm_sur_conv(const void* d_in, int cinp, const void* d_wpk, const float* d_bias, const void* d_zin,
-                 void* d_out, void* d_zout, int coutp, int epilogue, int N, int H, int W, void* stream) {
+                 void* d_out, void* d_zout, int coutp, int epilogue, int N, int H, int W, int elem, void* stream) {
   LBM_REQUIRE(N > 0 && H > 0 && W > 0 && W % 16 == 0, "conv: W must be a positive multiple of 16 (got %d)", W);
   LBM_REQUIRE((int64_t)N * H * W * 64 < ((int64_t)1 << 31), "conv: too many pixels");
   LBM_REQUIRE(epilogue >= 0 && epilogue <= 2, "conv: epilogue %d is not 0, 1 or 2 (see lbm_sur_apply_step)",
@@ -791,28 +840,28 @@ int lbm_sur_conv(const void* d_in, int cinp, const void* d_wpk, const float* d_b
   LBM_REQUIRE(epilogue == EPI_BWD || d_bias != nullptr, "conv: bias is NULL");
   LBM_REQUIRE(epilogue != EPI_BWD || d_zin != nullptr, "conv: the input-gradient epilogue needs z");
   hipStream_t s = (hipStream_t)stream;
-  const h16* in = (const h16*)d_in;
-  const h16* wp = (const h16*)d_wpk;
-  const h16* zi = (const h16*)d_zin;
-  h16* zo = (h16*)d_zout;
-  ApplyArgs ap{};
-#define SUR_CONV(CI, CO, E) \
-  if (cinp == CI && coutp == CO && epilogue == E) return conv_launch<CI, CO, E>(in, wp, d_bias, zi, d_out, zo, N, H, W, ap, s);
-  SUR_CONV(64, 64, EPI_ACT)
-  SUR_CONV(64, 32, EPI_ACT)
-  SUR_CONV(32, 32, EPI_ACT)
-  SUR_CONV(32, 64, EPI_LIN)
-  SUR_CONV(64, 64, EPI_LIN)
-  SUR_CONV(64, 64, EPI_BWD)
-  SUR_CONV(64, 32, EPI_BWD)
-  SUR_CONV(32, 32, EPI_BWD)
+  const ApplyArgs ap{};
+  return with_elem(elem, "conv", [&](auto tag) -> int {
+    using E = typename decltype(tag)::type;
+#define SUR_CONV(CI, CO, EP)                          \
+  if (cinp == CI && coutp == CO && epilogue == EP) \
+    return conv_launch<E, CI, CO, EP>(d_in, d_wpk, d_bias, d_zin, d_out, d_zout, N, H, W, ap, s);
+    SUR_CONV(64, 64, EPI_ACT)
+    SUR_CONV(64, 32, EPI_ACT)
+    SUR_CONV(32, 32, EPI_ACT)
+    SUR_CONV(32, 64, EPI_LIN)
+    SUR_CONV(64, 64, EPI_LIN)
+    SUR_CONV(64, 64, EPI_BWD)
+    SUR_CONV(64, 32, EPI_BWD)
+    SUR_CONV(32, 32, EPI_BWD)
 #undef SUR_CONV
-  LBM_REQUIRE(false, "conv: unsupported shape cinp=%d coutp=%d epilogue=%d", cinp, coutp, epilogue);
+    LBM_REQUIRE(false, "conv: unsupported shape cinp=%d coutp=%d epilogue=%d", cinp, coutp, epilogue);
+  });
 }
 
 int lbm_sur_apply_step(const void* d_in, int cinp, const void* d_wpk, const float* d_bias, float* d_state,
                        void* d_xnext, const float* dscale, const float* shift, const float* scale, int N, int H,
-                       int W, void* stream) {
+                       int W, int elem, void* stream) {
   LBM_REQUIRE(N > 0 && H > 0 && W > 0 && W % 16 == 0, "apply_step: W must be a positive multiple of 16");
   LBM_REQUIRE((int64_t)N * H * W * 64 < ((int64_t)1 << 31), "apply_step: too many pixels");
   LBM_REQUIRE(d_xnext != d_in, "apply_step: the next input must not alias the layer input");
@@ -822,15 +871,16 @@ int lbm_sur_apply_step(const void* d_in, int cinp, const void* d_wpk, const floa
   ap.shift = float4v{shift[0], shift[1], shift[2], 0.f};
   ap.inv_scale = float4v{1.f / scale[0], 1.f / scale[1], 1.f / scale[2], 0.f};
   ap.state = d_state;
-  ap.xnext = (h16*)d_xnext;
+  ap.xnext = d_xnext;
   hipStream_t s = (hipStream_t)stream;
-  if (cinp == 64)
-    return conv_launch<64, 64, EPI_APPLY>((const h16*)d_in, (const h16*)d_wpk, d_bias, nullptr, nullptr, nullptr, N,
-                                          H, W, ap, s);
-  if (cinp == 32)
-    return conv_launch<32, 64, EPI_APPLY>((const h16*)d_in, (const h16*)d_wpk, d_bias, nullptr, nullptr, nullptr, N,
-                                          H, W, ap, s);
-  LBM_REQUIRE(false, "apply_step: unsupported cinp=%d", cinp);
+  return with_elem(elem, "apply_step", [&](auto tag) -> int {
+    using E = typename decltype(tag)::type;
+    if (cinp == 64)
+      return conv_launch<E, 64, 64, EPI_APPLY>(d_in, d_wpk, d_bias, nullptr, nullptr, nullptr, N, H, W, ap, s);
+    if (cinp == 32)
+      return conv_launch<E, 32, 64, EPI_APPLY>(d_in, d_wpk, d_bias, nullptr, nullptr, nullptr, N, H, W, ap, s);
+    LBM_REQUIRE(false, "apply_step: unsupported cinp=%d", cinp);
+  });
 }
 
 int lbm_sur_run(const lbm_sur_net* net, float* d_state, const float* dscale, const float* shift,
@@ -847,19 +897,23 @@ int lbm_sur_run(const lbm_sur_net* net, float* d_state, const float* dscale, con
   ap.inv_scale = float4v{1.f / scale[0], 1.f / scale[1], 1.f / scale[2], 0.f};
   ap.state = d_state;
   hipStream_t s = (hipStream_t)stream;
-  h16* bufs[2] = {(h16*)net->act[0], (h16*)net->xalt};
-  for (int64_t k = 0; k < nsteps; ++k) {
-    // even steps read act[0] and write xalt, odd steps the reverse
-    const h16* xin = bufs[k & 1];
-    ap.xnext = bufs[(k + 1) & 1];
-    const int rc = net->hidden == 32 ? fused_launch<32>(net, xin, ap.xnext, ap, N, H, W, s)
-                                     : fused_launch<64>(net, xin, ap.xnext, ap, N, H, W, s);
-    if (rc) return rc;
-  }
+  void* bufs[2] = {net->act[0], net->xalt};
+  const int rc = with_elem(net->elem, "run", [&](auto tag) -> int {
+    using E = typename decltype(tag)::type;
+    for (int64_t k = 0; k < nsteps; ++k) {
+      // even steps read act[0] and write xalt, odd steps the reverse
+      ap.xnext = bufs[(k + 1) & 1];
+      const int r = net->hidden == 32 ? fused_launch<E, 32>(net, bufs[k & 1], ap, N, H, W, s)
+                                      : fused_launch<E, 64>(net, bufs[k & 1], ap, N, H, W, s);
+      if (r) return r;
+    }
+    return 0;
+  });
+  if (rc) return rc;
   if (nsteps & 1) {
     // the next input must end in act[0]
-    LBM_CHECK_HIP(hipMemcpyAsync(net->act[0], net->xalt, (size_t)N * H * W * 64 * sizeof(h16),
-                                 hipMemcpyDeviceToDevice, s));
+    LBM_CHECK_HIP(hipMemcpyAsync(net->act[0], net->xalt, (size_t)N * H * W * 64 * EB, hipMemcpyDeviceToDevice,
+                                 s));
   }
   return 0;
 }
@@ -867,7 +921,7 @@ int lbm_sur_run(const lbm_sur_net* net, float* d_state, const float* dscale, con
 int lbm_sur_wgrad_blocks(void) { return SUR_WGRAD_BLOCKS; }
 
 int lbm_sur_wgrad(const void* d_x, int cinp, const void* d_dz, int cdz, int N, int H, int W, float* d_partial,
-                  float* d_dwt, float* d_db, void* stream) {
+                  float* d_dwt, float* d_db, int elem, void* stream) {
   LBM_REQUIRE(N > 0 && H > 0 && W > 0 && W % WG_PIX == 0, "wgrad: W must be a positive multiple of %d (got %d)",
               WG_PIX, W);
   LBM_REQUIRE((int64_t)N * H * W * 64 < ((int64_t)1 << 31), "wgrad: too many pixels");
@@ -875,19 +929,24 @@ int lbm_sur_wgrad(const void* d_x, int cinp, const void* d_dz, int cdz, int N, i
   const int64_t len = 9 * (int64_t)cdz * cinp;
   float* pb = d_partial + (int64_t)nblk * len;
   hipStream_t s = (hipStream_t)stream;
-  const h16* x = (const h16*)d_x;
-  const h16* dz = (const h16*)d_dz;
   const dim3 grid(nblk, 9);
-  if (cinp == 64 && cdz == 64)
-    conv3x3_wgrad_kernel<64, 64><<<grid, 256, 0, s>>>(x, dz, d_partial, pb, N, H, W);
-  else if (cinp == 64 && cdz == 32)
-    conv3x3_wgrad_kernel<64, 32><<<grid, 256, 0, s>>>(x, dz, d_partial, pb, N, H, W);
-  else if (cinp == 32 && cdz == 32)
-    conv3x3_wgrad_kernel<32, 32><<<grid, 256, 0, s>>>(x, dz, d_partial, pb, N, H, W);
-  else if (cinp == 32 && cdz == 64)
-    conv3x3_wgrad_kernel<32, 64><<<grid, 256, 0, s>>>(x, dz, d_partial, pb, N, H, W);
-  else
-    LBM_REQUIRE(false, "wgrad: unsupported shape cinp=%d cdz=%d", cinp, cdz);
+  const int rc = with_elem(elem, "wgrad", [&](auto tag) -> int {
+    using E = typename decltype(tag)::type;
+    const E* x = (const E*)d_x;
+    const E* dz = (const E*)d_dz;
+    if (cinp == 64 && cdz == 64)
+      conv3x3_wgrad_kernel<E, 64, 64><<<grid, 256, 0, s>>>(x, dz, d_partial, pb, N, H, W);
+    else if (cinp == 64 && cdz == 32)
+      conv3x3_wgrad_kernel<E, 64, 32><<<grid, 256, 0, s>>>(x, dz, d_partial, pb, N, H, W);
+    else if (cinp == 32 && cdz == 32)
+      conv3x3_wgrad_kernel<E, 32, 32><<<grid, 256, 0, s>>>(x, dz, d_partial, pb, N, H, W);
+    else if (cinp == 32 && cdz == 64)
+      conv3x3_wgrad_kernel<E, 32, 64><<<grid, 256, 0, s>>>(x, dz, d_partial, pb, N, H, W);
+    else
+      LBM_REQUIRE(false, "wgrad: unsupported shape cinp=%d cdz=%d", cinp, cdz);
+    return 0;
+  });
+  if (rc) return rc;
   LBM_CHECK_HIP(hipGetLastError());
   sum_partials_kernel<<<blocks_for(len), 256, 0, s>>>(d_partial, nblk, (int)len, d_dwt);
   LBM_CHECK_HIP(hipGetLastError());
@@ -897,10 +956,15 @@ int lbm_sur_wgrad(const void* d_x, int cinp, const void* d_dz, int cdz, int N, i
 }
 
 int lbm_sur_mse(const float* d_y, const float* d_t, int64_t npix, float gscale, void* d_dy, double* d_partial,
-                double* h_loss, void* stream) {
+                double* h_loss, int elem, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int64_t nq = npix * 16;  // fine pixels
-  mse_kernel<<<256, 256, 0, s>>>(d_y, d_t, nq, gscale, (h16*)d_dy, d_partial);
+  const int rc = with_elem(elem, "mse", [&](auto tag) -> int {
+    using E = typename decltype(tag)::type;
+    mse_kernel<E><<<256, 256, 0, s>>>(d_y, d_t, nq, gscale, (E*)d_dy, d_partial);
+    return 0;
+  });
+  if (rc) return rc;
   LBM_CHECK_HIP(hipGetLastError());
   if (h_loss) {
     double tmp[256];
@@ -913,11 +977,16 @@ int lbm_sur_mse(const float* d_y, const float* d_t, int64_t npix, float gscale, 
   return 0;
 }
 
-int lbm_sur_pack(const float* d_w, int cin, int cout, int cinp, int coutp, void* d_afwd, void* d_abwd,
+int lbm_sur_pack(const float* d_w, int cin, int cout, int cinp, int coutp, void* d_afwd, void* d_abwd, int elem,
                  void* stream) {
   LBM_REQUIRE(cin <= cinp && cout <= coutp && cinp <= 64 && coutp <= 64, "pack: channel counts out of range");
-  pack_kernel<<<blocks_for(coutp * 9 * cinp), 256, 0, (hipStream_t)stream>>>(d_w, cin, cout, cinp, coutp,
-                                                                            (h16*)d_afwd, (h16*)d_abwd);
+  const int rc = with_elem(elem, "pack", [&](auto tag) -> int {
+    using E = typename decltype(tag)::type;
+    pack_kernel<E><<<blocks_for(coutp * 9 * cinp), 256, 0, (hipStream_t)stream>>>(d_w, cin, cout, cinp, coutp,
+                                                                                 (E*)d_afwd, (E*)d_abwd);
+    return 0;
+  });
+  if (rc) return rc;
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -934,12 +1003,17 @@ int lbm_sur_adam(float* d_w, float* d_b, float* d_m, float* d_v, const float* d_
 }
 
 int lbm_sur_make_input(const float* d_state, int N, int Hf, int Wf, const float* shift, const float* scale,
-                       void* d_xin, void* stream) {
+                       void* d_xin, int elem, void* stream) {
   LBM_REQUIRE(N > 0 && Hf % 4 == 0 && Wf % 4 == 0, "make_input: the fine grid must be a multiple of 4");
   float4v sh = {shift[0], shift[1], shift[2], 0.f};
   float4v is = {1.f / scale[0], 1.f / scale[1], 1.f / scale[2], 0.f};
-  make_input_kernel<<<blocks_for((int64_t)N * Hf * Wf), 256, 0, (hipStream_t)stream>>>(d_state, N, Hf, Wf, sh, is,
-                                                                                      (h16*)d_xin);
+  const int rc = with_elem(elem, "make_input", [&](auto tag) -> int {
+    using E = typename decltype(tag)::type;
+    make_input_kernel<E><<<blocks_for((int64_t)N * Hf * Wf), 256, 0, (hipStream_t)stream>>>(d_state, N, Hf, Wf, sh,
+                                                                                           is, (E*)d_xin);
+    return 0;
+  });
+  if (rc) return rc;
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
 }

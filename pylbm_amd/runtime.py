@@ -101,19 +101,19 @@ _FP = ctypes.POINTER(ctypes.c_float)
 
 class LbmSurNet(ctypes.Structure):
     _fields_ = [("act", ctypes.c_void_p * 4), ("xalt", ctypes.c_void_p), ("wpk", ctypes.c_void_p * 4),
-                ("bias", ctypes.c_void_p * 4), ("hidden", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+                ("bias", ctypes.c_void_p * 4), ("hidden", ctypes.c_int32), ("elem", ctypes.c_int32)]
 
 
 SURROGATE_SIGNATURES = {
     "lbm_sur_run": ([ctypes.POINTER(LbmSurNet), _P, _FP, _FP, _FP, _I64, _I, _I, _I, _P], _I),
-    "lbm_sur_conv": ([_P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P], _I),
-    "lbm_sur_apply_step": ([_P, _I, _P, _P, _P, _P, _FP, _FP, _FP, _I, _I, _I, _P], _I),
+    "lbm_sur_conv": ([_P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
+    "lbm_sur_apply_step": ([_P, _I, _P, _P, _P, _P, _FP, _FP, _FP, _I, _I, _I, _I, _P], _I),
     "lbm_sur_wgrad_blocks": ([], _I),
-    "lbm_sur_wgrad": ([_P, _I, _P, _I, _I, _I, _I, _P, _P, _P, _P], _I),
-    "lbm_sur_mse": ([_P, _P, _I64, _F, _P, _P, ctypes.POINTER(_D), _P], _I),
-    "lbm_sur_pack": ([_P, _I, _I, _I, _I, _P, _P, _P], _I),
+    "lbm_sur_wgrad": ([_P, _I, _P, _I, _I, _I, _I, _P, _P, _P, _I, _P], _I),
+    "lbm_sur_mse": ([_P, _P, _I64, _F, _P, _P, ctypes.POINTER(_D), _I, _P], _I),
+    "lbm_sur_pack": ([_P, _I, _I, _I, _I, _P, _P, _I, _P], _I),
     "lbm_sur_adam": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _F, _F, _F, _I, _P], _I),
-    "lbm_sur_make_input": ([_P, _I, _I, _I, _FP, _FP, _P, _P], _I),
+    "lbm_sur_make_input": ([_P, _I, _I, _I, _FP, _FP, _P, _I, _P], _I),
     "lbm_sur_make_target": ([_P, _P, _I, _I, _I, _FP, _P, _P], _I),
     "lbm_sur_moments_to_state": ([_P, ctypes.POINTER(LbmLayout), _I, _I, _I, _I, _I, _P, _P], _I),
     "lbm_sur_state_to_moments": ([_P, _I, _I, ctypes.POINTER(LbmLayout), _I, _I, _I, _P, _P], _I),

@@ -6,9 +6,9 @@ fine state is unshuffled into coarse pixels of 64 channels (16 fine pixels x 4
 fields), so the convolutions run on a grid 16 times smaller and see 16 fine
 cells in every direction (farther than sound travels in K = 16 steps); the
 network is 64 -> C -> C -> C -> 64 channels with SiLU, C = 64 by default.
-Training and inference run on the project's own HIP kernels (FP16 MFMA
-operands, FP32 accumulation, FP32 master weights, Adam); training data are
-trajectories the HIP lattice Boltzmann run itself produces.
+Training and inference run on the project's own HIP kernels (FP16 or BF16
+MFMA operands, FP32 accumulation, FP32 master weights, Adam); training data
+are trajectories the HIP lattice Boltzmann run itself produces.
 
 The moments are read through the same route as ``Simulation.m``
 (pylbm/simulation.py:215-224: f2m over the halo domain) and the lattice
@@ -36,6 +36,8 @@ from .simulation import Simulation
 PATCH = 4
 IN_CH = 64
 EPI_ACT, EPI_LIN, EPI_BWD = 0, 1, 2
+# 16-bit element type of activations, gradients and packed weights (LBM_SUR_F16 / _BF16)
+ELEMS = {"float16": 0, "bfloat16": 1}
 
 
 def channels(hidden=64):
@@ -63,11 +65,15 @@ def init_params(seed=0, hidden=64, scale_last=0.1):
 
 
 class SurrogateNet:
-    """Device-resident network for batches of up to ``batch`` fine fields of Hf x Wf."""
+    """Device-resident network for batches of up to ``batch`` fine fields of Hf x Wf;
+    ``elem`` ('float16' or 'bfloat16') is the type of its 16-bit MFMA operands."""
 
-    def __init__(self, rt, Hf, Wf, batch=1, hidden=64, train=True, params=None, seed=0):
+    def __init__(self, rt, Hf, Wf, batch=1, hidden=64, train=True, params=None, seed=0, elem="float16"):
         if Hf % PATCH or Wf % (PATCH * 32):
             raise ValueError("the surrogate needs Hf % 4 == 0 and Wf % 128 == 0 (got %dx%d)" % (Hf, Wf))
+        if elem not in ELEMS:
+            raise ValueError("elem must be one of %s (got %r)" % (sorted(ELEMS), elem))
+        self.elem_name, self.elem = elem, ELEMS[elem]
         self.rt, self.lib, self.batch, self.train = rt, rt.lib, batch, train
         self.stream = rt.stream  # every launch of the network goes to this stream
         self.Hf, self.Wf = Hf, Wf
@@ -152,14 +158,14 @@ class SurrogateNet:
         for i in range(4):
             abwd = self.abwd[i] if self.train else None
             self.rt.check(self.lib.lbm_sur_pack(ctypes.c_void_p(self.w[i]), ch[i], ch[i + 1], ch[i], ch[i + 1],
-                                                ctypes.c_void_p(self.afwd[i]), ctypes.c_void_p(abwd),
+                                                ctypes.c_void_p(self.afwd[i]), ctypes.c_void_p(abwd), self.elem,
                                                 self.stream), "lbm_sur_pack")
 
     # ------------------------------------------------------------ passes
     def _conv(self, src, cinp, wpk, bias, zin, out, zout, coutp, epi, n):
         self.rt.check(self.lib.lbm_sur_conv(ctypes.c_void_p(src), cinp, ctypes.c_void_p(wpk),
                                             ctypes.c_void_p(bias), ctypes.c_void_p(zin), ctypes.c_void_p(out),
-                                            ctypes.c_void_p(zout), coutp, epi, n, self.H, self.W,
+                                            ctypes.c_void_p(zout), coutp, epi, n, self.H, self.W, self.elem,
                                             self.stream), "lbm_sur_conv")
 
     def hidden_layers(self, n=1, keep=False):
@@ -186,7 +192,7 @@ class SurrogateNet:
         self.rt.check(self.lib.lbm_sur_apply_step(ctypes.c_void_p(self.act[3]), self.hidden,
                                                   ctypes.c_void_p(self.afwd[3]), ctypes.c_void_p(self.b[3]),
                                                   ctypes.c_void_p(d_state), ctypes.c_void_p(self.xin), dscale,
-                                                  shift, scale, n, self.H, self.W, self.stream),
+                                                  shift, scale, n, self.H, self.W, self.elem, self.stream),
                       "lbm_sur_apply_step")
 
     def run(self, d_state, norm_args, nsteps):
@@ -200,6 +206,7 @@ class SurrogateNet:
                 c.bias[i] = self.b[i]
             c.xalt = self.xalt
             c.hidden = self.hidden
+            c.elem = self.elem
             self._cnet = c
         self.rt.check(self.lib.lbm_sur_run(ctypes.byref(self._cnet), ctypes.c_void_p(d_state), dscale, shift, scale,
                                            int(nsteps), 1, self.H, self.W, self.stream), "lbm_sur_run")
@@ -209,7 +216,8 @@ class SurrogateNet:
         out = ctypes.c_double(0.0)
         self.rt.check(self.lib.lbm_sur_mse(ctypes.c_void_p(self.y), ctypes.c_void_p(self.tgt), n * self.H * self.W,
                                            float(gscale), ctypes.c_void_p(self.dy), ctypes.c_void_p(self.lpart),
-                                           ctypes.byref(out) if sync else None, self.stream), "lbm_sur_mse")
+                                           ctypes.byref(out) if sync else None, self.elem, self.stream),
+                      "lbm_sur_mse")
         return out.value if sync else None
 
     def backward(self, n):
@@ -219,15 +227,15 @@ class SurrogateNet:
         for i in range(3, -1, -1):
             self.rt.check(lib.lbm_sur_wgrad(ctypes.c_void_p(self.act[i]), ch[i], ctypes.c_void_p(grad_in), ch[i + 1],
                                             n, self.H, self.W, ctypes.c_void_p(self.partial),
-                                            ctypes.c_void_p(self.dwt[i]), ctypes.c_void_p(self.db[i]), st),
-                          "lbm_sur_wgrad")
+                                            ctypes.c_void_p(self.dwt[i]), ctypes.c_void_p(self.db[i]),
+                                            self.elem, st), "lbm_sur_wgrad")
             if i > 0:
                 out = self.g[i % 2]
                 self._conv(grad_in, ch[i + 1], self.abwd[i], None, self.z[i - 1], out, None, ch[i], EPI_BWD, n)
                 grad_in = out
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, eps_scale=1.0):
-        """One Adam step on every layer, then repack the FP16 operands."""
+        """One Adam step on every layer, then repack the 16-bit operands."""
         self.step += 1
         ch = self.ch
         for i in range(4):
@@ -458,8 +466,8 @@ class Trainer:
             if pf:
                 self.rt.d2d(sb, d.state_ptr(slot, t - 1), d.stride)
                 self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(sb), 1, lat.H, lat.W, self._shift,
-                                                          self._scale, ctypes.c_void_p(net.xin), self.rt.stream),
-                              "lbm_sur_make_input")
+                                                          self._scale, ctypes.c_void_p(net.xin), net.elem,
+                                                          self.rt.stream), "lbm_sur_make_input")
                 net.run(sb, norm_args, 1)
             elif self.noise > 0:
                 self.rt.d2d(sb, d.state_ptr(slot, t), d.stride)
@@ -473,7 +481,7 @@ class Trainer:
             s1 = d.state_ptr(slot, t + 1)
             self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(s0), 1, lat.H, lat.W, self._shift,
                                                       self._scale, ctypes.c_void_p(net.xin + b * npix * 8),
-                                                      self.rt.stream), "lbm_sur_make_input")
+                                                      net.elem, self.rt.stream), "lbm_sur_make_input")
             self.rt.check(self.lib.lbm_sur_make_target(ctypes.c_void_p(s0), ctypes.c_void_p(s1), 1, lat.H, lat.W,
                                                        self._dscale, ctypes.c_void_p(net.tgt + b * npix * 16),
                                                        self.rt.stream), "lbm_sur_make_target")
@@ -540,8 +548,8 @@ class SurrogateRun:
 
     def _make_input(self, d_state):
         self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(d_state), 1, self.lat.H, self.lat.W, self._shift,
-                                                  self._scale, ctypes.c_void_p(self.net.xin), self.rt.stream),
-                      "lbm_sur_make_input")
+                                                  self._scale, ctypes.c_void_p(self.net.xin), self.net.elem,
+                                                  self.rt.stream), "lbm_sur_make_input")
 
     def _net_step(self, d_state):
         """d_state += increment(net(xin)); xin must hold d_state's input and then holds the next one."""
@@ -609,7 +617,8 @@ class SurrogateRun:
         self._record(ev["free"][0], B)
         self._wait(A, ev["free"][0])
         self.rt.check(self.lib.lbm_sur_make_input(ctypes.c_void_p(self.state), 1, lat.H, lat.W, self._shift,
-                                                  self._scale, ctypes.c_void_p(self.net.xin), A), "make_input")
+                                                  self._scale, ctypes.c_void_p(self.net.xin), self.net.elem, A),
+                      "make_input")
         pending = None
         i = j = 0
 
@@ -737,7 +746,8 @@ class SurrogateRun:
 
 
 def train_surrogate(lattice, ntraj=200, steps=2000, K=16, pool=8, batch=8, steps_per_traj=100, lr=2e-3,
-                    lr_min=1e-5, seed=0, kband=(4.0, 16.0), hidden=32, pushforward=0.0, noise=0.002, log=None):
+                    lr_min=1e-5, seed=0, kband=(4.0, 16.0), hidden=32, pushforward=0.0, noise=0.002, log=None,
+                    elem="float16"):
     """Generate ntraj trajectories with the HIP lattice Boltzmann run and train a network on them.
 
     Returns (net, norm, info). Trajectory seeds are seed * 100000 + j."""
@@ -746,7 +756,7 @@ def train_surrogate(lattice, ntraj=200, steps=2000, K=16, pool=8, batch=8, steps
     t0 = time.perf_counter()
     slot = data.generate(seed * 100000)
     norm = Normalization.from_states(data.download(slot, every=4))
-    net = SurrogateNet(rt, lattice.H, lattice.W, batch=batch, hidden=hidden, train=True, seed=seed)
+    net = SurrogateNet(rt, lattice.H, lattice.W, batch=batch, hidden=hidden, train=True, seed=seed, elem=elem)
     trainer = Trainer(net, data, norm, seed=seed, pushforward=pushforward, noise=noise)
     total = ntraj * steps_per_traj
 
@@ -771,7 +781,7 @@ def train_surrogate(lattice, ntraj=200, steps=2000, K=16, pool=8, batch=8, steps
     data.free()
     trainer.free()
     info = {"trajectories": ntraj, "steps_per_trajectory": steps, "K": K, "optimizer_steps": net.step,
-            "batch": batch, "hidden": hidden, "pushforward": pushforward, "noise": noise,
+            "batch": batch, "hidden": hidden, "elem": elem, "pushforward": pushforward, "noise": noise,
             "final_loss": trainer.losses[-1][1], "generate_s": t_gen, "train_s": t_train,
             "losses": trainer.losses}
     return net, norm, info

@@ -157,6 +157,7 @@ def _run_overlapped(nintervals, je, bad=()):
     run.net = _Net(bufs, log)
     run.net.stream = "B"
     run.net.xin = 0
+    run.net.elem = 0
     run.lat = _Lat(bufs, 16, log)
     run.lat.H = run.lat.W = 4
     run.state, run.pred, run.lpart, run.jout, run.hdrift = 1, 2, 50, 40, 1000
