@@ -82,11 +82,14 @@ typedef struct lbm_sur_net {
 
 /* nsteps inference steps without returning to the host, one launch per step: a
  * workgroup computes all 4 layers of a 4 x 16 coarse-pixel tile (halo recomputed,
- * activations in LDS), adds the increment to d_state and writes the next input.
- * The input alternates between act[0] and xalt; on return act[0] holds the input
- * of the new d_state.  act[0] must hold the input of d_state on entry.  The coarse
- * grid must be a multiple of 4 x 16. */
-int lbm_sur_run(const lbm_sur_net* net, float* d_state, const float* dscale, const float* shift,
+ * activations in LDS), adds the increment to the state and writes the next input.
+ * The first step reads d_src (NULL: d_state) and writes d_state; later steps update
+ * d_state in place (d_src == d_state is in place throughout; nsteps == 0 copies
+ * d_src to d_state).  act[0] must hold the network input of d_src on entry; on
+ * return it holds the input of the new d_state: after an odd nsteps the call
+ * exchanges act[0] and xalt in *net rather than copying.  The coarse grid must be
+ * a multiple of 4 x 16. */
+int lbm_sur_run(lbm_sur_net* net, const float* d_src, float* d_state, const float* dscale, const float* shift,
                 const float* scale, int64_t nsteps, int N, int H, int W, void* stream);
 
 /* number of workgroups per tap (partial sums) lbm_sur_wgrad uses */

@@ -74,7 +74,8 @@ __device__ __forceinline__ void staged_copy(Ld ld, St st) {
 
 struct ApplyArgs {
   float4v dscale, shift, inv_scale;
-  float* state;  // fine FP32 [N][4H][4W][4]
+  const float* state_in;  // fine FP32 [N][4H][4W][4] the increment is added to
+  float* state;           // where the result goes (may equal state_in)
   void* xnext;   // next network input, coarse E [N][H][W][64]
 };
 
@@ -194,7 +195,7 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const E* __restrict__ in, 
       const int Wf = 4 * W;
       const int64_t fp = ((int64_t)n * 4 * H + 4 * y + py) * Wf + 4 * (x0 + r) + pxo;
       float4v s;
-      __builtin_memcpy(&s, ap.state + fp * 4, 16);
+      __builtin_memcpy(&s, ap.state_in + fp * 4, 16);
       half4 xb;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
@@ -209,7 +210,7 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const E* __restrict__ in, 
 }
 
 // ------------------------------------------------------- fused inference step
-// One launch per inference step: a workgroup computes the 4 layers for a core tile
+// One launch per inference step: a workgroup (FUSED_WAVES waves) computes the 4 layers for a core tile
 // of FT_Y x FT_X coarse pixels, recomputing the 3-pixel halo each hidden layer needs
 // (layer l produces a (FT_Y + 2 (3 - l)) x (FT_X + 2 (3 - l)) region).  Activations
 // stay in LDS.  Input and weights are copied into LDS asynchronously (LDS DMA); with
@@ -221,6 +222,9 @@ constexpr int FT_Y = 4, FT_X = 16;
 #endif
 #ifndef FUSED_PREFETCH
 #define FUSED_PREFETCH 2   // K steps of LDS operands read ahead of the MFMAs
+#endif
+#ifndef FUSED_WAVES
+#define FUSED_WAVES 16     // waves per workgroup (one workgroup per CU: its LDS use)
 #endif
 
 template <int C>
@@ -247,13 +251,13 @@ struct LayerW {
 
 // Asynchronous copy into LDS (gfx950 global_load_lds_dwordx4): NU 16-byte units,
 // unit u lands at dst + 8 u elements and is read from src_of(u).  A wave issues the
-// units [64 k, 64 k + 64) for k = wave, wave + 4, ...; the caller waits
+// units [64 k, 64 k + 64) for k = wave, wave + FUSED_WAVES, ...; the caller waits
 // (s_waitcnt + barrier) before reading dst.
 template <int NU, typename E, typename F>
 __device__ __forceinline__ void dma_units(E* dst, F src_of) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll 4
-  for (int k = wv; k * 64 < NU; k += 4) {
+  for (int k = wv; k * 64 < NU; k += FUSED_WAVES) {
     const int u = k * 64 + lane;
     if (u < NU)
       __builtin_amdgcn_global_load_lds((const void*)src_of(u),
@@ -291,12 +295,12 @@ __device__ __forceinline__ void fused_layer(const E* __restrict__ src, E* __rest
   constexpr int MT = COUT / 16;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
-  // TP = 2: a wave takes a pair of tiles (2 j, 2 j + 1) per pass, pairs j = wv, wv + 4,
-  // ...: each weight read feeds two MFMAs and the MFMA pipe has 2 MT independent
-  // chains; TP = 1: one tile per pass (more waves busy on small layers)
+  // TP = 2: a wave takes a pair of tiles (2 j, 2 j + 1) per pass, pairs j = wv,
+  // wv + FUSED_WAVES, ...: each weight read feeds two MFMAs and the MFMA pipe has 2 MT
+  // independent chains; TP = 1: one tile per pass (more waves busy on small layers)
   constexpr int TP = ((FUSED_PAIR_MASK >> LAYER) & 1) ? 2 : 1;
   constexpr int GROUPS = (TILES + TP - 1) / TP;
-  for (int j = wv; j < GROUPS; j += 4) {
+  for (int j = wv; j < GROUPS; j += FUSED_WAVES) {
     int pcs[2], prs[2], pqs[2];
     bool st[2];
 #pragma unroll
@@ -362,7 +366,7 @@ __device__ __forceinline__ void fused_layer(const E* __restrict__ src, E* __rest
           const int py = c0 >> 4, pxo = (c0 >> 2) & 3;
           const int64_t fp = ((int64_t)n * 4 * H + 4 * y + py) * (4 * W) + 4 * x + pxo;
           float4v sv;
-          __builtin_memcpy(&sv, ap.state + fp * 4, 16);
+          __builtin_memcpy(&sv, ap.state_in + fp * 4, 16);
           half4 xb;
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
@@ -379,7 +383,7 @@ __device__ __forceinline__ void fused_layer(const E* __restrict__ src, E* __rest
 }
 
 template <typename E, int C>
-__global__ void __launch_bounds__(256) fused_step_kernel(const E* __restrict__ xin, const E* __restrict__ w0,
+__global__ void __launch_bounds__(64 * FUSED_WAVES) fused_step_kernel(const E* __restrict__ xin, const E* __restrict__ w0,
                                                          const E* __restrict__ w1, const E* __restrict__ w2,
                                                          const E* __restrict__ w3, const float* __restrict__ b0,
                                                          const float* __restrict__ b1, const float* __restrict__ b2,
@@ -396,13 +400,16 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const E* __restrict__ x
   const int n = blockIdx.x / (tx * ty);
   const int y0 = ((blockIdx.x / tx) % ty) * FT_Y, x0 = (blockIdx.x % tx) * FT_X;
   // layer-0 input region: rows y0 - 4 .. y0 + FT_Y + 3, columns x0 - 4 .. x0 + FT_X + 3
-  // (periodic), 9 units per pixel (8 of channels + 1 of padding)
+  // (periodic: one wrap suffices, since H >= FT_Y and W >= FT_X), 9 units per pixel
+  // (8 of channels + 1 of padding)
   {
     const E* img = xin + (int64_t)n * H * W * 64;
     dma_units<G::R_IN * G::W_IN * 9>(bufA, [&](int u) {
       const int q = u / 9, c = u % 9;
       const int rr = q / G::W_IN, cc = q % G::W_IN;
-      const int yy = (y0 + rr - 4 + 4 * H) % H, xx = (x0 + cc - 4 + 4 * W) % W;
+      int yy = y0 + rr - 4, xx = x0 + cc - 4;
+      yy += yy < 0 ? H : (yy >= H ? -H : 0);
+      xx += xx < 0 ? W : (xx >= W ? -W : 0);
       return img + ((int64_t)yy * W + xx) * 64 + (c < 8 ? c : 7) * 8;
     });
   }
@@ -818,7 +825,7 @@ int fused_launch(const lbm_sur_net* net, const void* xin, const ApplyArgs& ap, i
     attr = true;
   }
   const unsigned blocks = (unsigned)((int64_t)N * (H / FT_Y) * (W / FT_X));
-  fused_step_kernel<E, C><<<blocks, 256, G::LDS, s>>>(
+  fused_step_kernel<E, C><<<blocks, 64 * FUSED_WAVES, G::LDS, s>>>(
       (const E*)xin, (const E*)net->wpk[0], (const E*)net->wpk[1], (const E*)net->wpk[2], (const E*)net->wpk[3],
       net->bias[0], net->bias[1], net->bias[2], net->bias[3], N, H, W, ap);
   LBM_CHECK_HIP(hipGetLastError());
@@ -870,6 +877,7 @@ int lbm_sur_apply_step(const void* d_in, int cinp, const void* d_wpk, const floa
   ap.dscale = float4v{dscale[0], dscale[1], dscale[2], 0.f};
   ap.shift = float4v{shift[0], shift[1], shift[2], 0.f};
   ap.inv_scale = float4v{1.f / scale[0], 1.f / scale[1], 1.f / scale[2], 0.f};
+  ap.state_in = d_state;
   ap.state = d_state;
   ap.xnext = d_xnext;
   hipStream_t s = (hipStream_t)stream;
@@ -883,7 +891,7 @@ int lbm_sur_apply_step(const void* d_in, int cinp, const void* d_wpk, const floa
   });
 }
 
-int lbm_sur_run(const lbm_sur_net* net, float* d_state, const float* dscale, const float* shift,
+int lbm_sur_run(lbm_sur_net* net, const float* d_src, float* d_state, const float* dscale, const float* shift,
                 const float* scale, int64_t nsteps, int N, int H, int W, void* stream) {
   LBM_REQUIRE(net != nullptr && (net->hidden == 32 || net->hidden == 64), "run: hidden must be 32 or 64");
   LBM_REQUIRE(nsteps >= 0, "run: nsteps < 0");
@@ -891,17 +899,26 @@ int lbm_sur_run(const lbm_sur_net* net, float* d_state, const float* dscale, con
               "run: the coarse grid must be a multiple of %dx%d (got %dx%d)", FT_Y, FT_X, H, W);
   LBM_REQUIRE((int64_t)N * H * W * 64 < ((int64_t)1 << 31), "run: too many pixels");
   LBM_REQUIRE(net->xalt != nullptr && net->xalt != net->act[0], "run: needs a second input buffer xalt");
+  LBM_REQUIRE(d_state != nullptr, "run: d_state is NULL");
+  if (d_src == nullptr) d_src = d_state;
   ApplyArgs ap;
   ap.dscale = float4v{dscale[0], dscale[1], dscale[2], 0.f};
   ap.shift = float4v{shift[0], shift[1], shift[2], 0.f};
   ap.inv_scale = float4v{1.f / scale[0], 1.f / scale[1], 1.f / scale[2], 0.f};
   ap.state = d_state;
   hipStream_t s = (hipStream_t)stream;
+  if (nsteps == 0) {
+    if (d_src != d_state)
+      LBM_CHECK_HIP(hipMemcpyAsync(d_state, d_src, (size_t)N * H * W * 16 * 16, hipMemcpyDeviceToDevice, s));
+    return 0;
+  }
   void* bufs[2] = {net->act[0], net->xalt};
   const int rc = with_elem(net->elem, "run", [&](auto tag) -> int {
     using E = typename decltype(tag)::type;
     for (int64_t k = 0; k < nsteps; ++k) {
-      // even steps read act[0] and write xalt, odd steps the reverse
+      // step 0 reads d_src, later steps update d_state in place; even steps read
+      // the input from act[0] and write the next one to xalt, odd steps the reverse
+      ap.state_in = k == 0 ? d_src : d_state;
       ap.xnext = bufs[(k + 1) & 1];
       const int r = net->hidden == 32 ? fused_launch<E, 32>(net, bufs[k & 1], ap, N, H, W, s)
                                       : fused_launch<E, 64>(net, bufs[k & 1], ap, N, H, W, s);
@@ -911,9 +928,9 @@ int lbm_sur_run(const lbm_sur_net* net, float* d_state, const float* dscale, con
   });
   if (rc) return rc;
   if (nsteps & 1) {
-    // the next input must end in act[0]
-    LBM_CHECK_HIP(hipMemcpyAsync(net->act[0], net->xalt, (size_t)N * H * W * 64 * EB, hipMemcpyDeviceToDevice,
-                                 s));
+    // the next input is in xalt: exchange the two buffers instead of copying
+    net->act[0] = bufs[1];
+    net->xalt = bufs[0];
   }
   return 0;
 }

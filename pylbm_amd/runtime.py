@@ -105,7 +105,7 @@ class LbmSurNet(ctypes.Structure):
 
 
 SURROGATE_SIGNATURES = {
-    "lbm_sur_run": ([ctypes.POINTER(LbmSurNet), _P, _FP, _FP, _FP, _I64, _I, _I, _I, _P], _I),
+    "lbm_sur_run": ([ctypes.POINTER(LbmSurNet), _P, _P, _FP, _FP, _FP, _I64, _I, _I, _I, _P], _I),
     "lbm_sur_conv": ([_P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
     "lbm_sur_apply_step": ([_P, _I, _P, _P, _P, _P, _FP, _FP, _FP, _I, _I, _I, _I, _P], _I),
     "lbm_sur_wgrad_blocks": ([], _I),

@@ -85,7 +85,9 @@ class SurrogateNet:
         self._bufs = []
         al = self._alloc
         self.xin = al(npix * IN_CH * 2)
-        self.xalt = al(self.H * self.W * IN_CH * 2)       # second input buffer of the fused inference step
+        # second input buffer of the fused inference step; run() exchanges it with xin
+        # after an odd number of steps, so it has xin's size
+        self.xalt = al(npix * IN_CH * 2)
         self.act = [self.xin] + [al(npix * hidden * 2) for _ in range(3)]
         self.y = al(npix * IN_CH * 4)
         self.w = [al(9 * ch[i] * ch[i + 1] * 4) for i in range(4)]
@@ -195,8 +197,11 @@ class SurrogateNet:
                                                   shift, scale, n, self.H, self.W, self.elem, self.stream),
                       "lbm_sur_apply_step")
 
-    def run(self, d_state, norm_args, nsteps):
-        """nsteps inference steps of one field in one library call; xin must hold d_state's input."""
+    def run(self, d_state, norm_args, nsteps, src=None):
+        """nsteps inference steps of one field in one library call: d_state = the state
+        nsteps steps after src (default d_state); xin must hold src's input and then
+        holds the input of the new d_state (xin and xalt trade places after an odd
+        nsteps)."""
         dscale, shift, scale = norm_args
         if not hasattr(self, "_cnet"):
             c = rtm.LbmSurNet()
@@ -208,8 +213,12 @@ class SurrogateNet:
             c.hidden = self.hidden
             c.elem = self.elem
             self._cnet = c
-        self.rt.check(self.lib.lbm_sur_run(ctypes.byref(self._cnet), ctypes.c_void_p(d_state), dscale, shift, scale,
-                                           int(nsteps), 1, self.H, self.W, self.stream), "lbm_sur_run")
+        c = self._cnet
+        self.rt.check(self.lib.lbm_sur_run(ctypes.byref(c), ctypes.c_void_p(src), ctypes.c_void_p(d_state), dscale,
+                                           shift, scale, int(nsteps), 1, self.H, self.W, self.stream), "lbm_sur_run")
+        if c.act[0] != self.xin:
+            self.xin, self.xalt = c.act[0], c.xalt
+            self.act[0] = self.xin
 
     def loss(self, n, gscale=1.0, sync=True):
         """Masked MSE of y against tgt; writes dY = 2 gscale (y - t). Returns the loss if sync."""
@@ -722,9 +731,9 @@ class SurrogateRun:
                 self.net.run(self.state, norm_args, nsur)
                 i += nsur
                 continue
-            # the plain run computes this interval from the surrogate state
-            self.rt.d2d(self.pred, self.state, nbytes)
-            self.net.run(self.pred, norm_args, 1)
+            # the plain run computes this interval from the surrogate state; the
+            # network's prediction for it goes to pred
+            self.net.run(self.pred, norm_args, 1, src=self.state)
             lat.restart(self.state)
             lat.run(K)
             lat.read_state(self.state)

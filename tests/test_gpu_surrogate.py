@@ -244,6 +244,48 @@ def test_fused_run_matches_layered_steps(rt, hidden, steps, elem):
     net.free()
 
 
+@pytest.mark.parametrize("steps", [1, 2, 3])
+def test_run_out_of_place_and_input_exchange(rt, steps):
+    """lbm_sur_run from a separate source state equals the in-place run bit for bit,
+    leaves the source untouched, and (after an odd step count, by exchanging xin and
+    xalt) leaves the next input in xin; nsteps = 0 copies the source."""
+    from pylbm_amd.surrogate import _f3
+    Hf, Wf = 32, 128
+    net, _ = _net(rt, 1, Hf, Wf, hidden=32, seed=51, last=1.0)
+    rng = np.random.default_rng(52)
+    s0 = np.zeros((Hf, Wf, 4), np.float32)
+    s0[..., 0] = 1.0 + 1e-3 * rng.normal(size=(Hf, Wf))
+    s0[..., 1:3] = 0.05 * rng.normal(size=(Hf, Wf, 2))
+    na = (_f3((1e-3, 2e-3, 2e-3)), _f3((1.0, 0.0, 0.0)), _f3((2e-3, 0.05, 0.05)))
+    da, db, dc = (rt.malloc(s0.nbytes) for _ in range(3))
+
+    def load_input(d):
+        rt.check(rt.lib.lbm_sur_make_input(ctypes.c_void_p(d), 1, Hf, Wf, na[1], na[2], ctypes.c_void_p(net.xin),
+                                           net.elem, rt.stream), "make_input")
+
+    rt.h2d(da, s0)
+    load_input(da)
+    net.run(da, na, steps)
+    sa, xa = _download(rt, da, s0.shape, np.float32), _download(rt, net.xin, (net.H, net.W, 64), H16)
+    rt.h2d(db, s0)
+    rt.h2d(dc, np.full_like(s0, np.nan))
+    xin_before = net.xin
+    load_input(db)
+    net.run(dc, na, steps, src=db)
+    assert (net.xin != xin_before) == bool(steps & 1)
+    assert np.array_equal(_download(rt, dc, s0.shape, np.float32), sa)
+    assert np.array_equal(_download(rt, db, s0.shape, np.float32), s0)
+    assert np.array_equal(_download(rt, net.xin, (net.H, net.W, 64), H16), xa)
+    # the next input really is the input of the new state
+    load_input(da)
+    assert np.array_equal(_download(rt, net.xin, (net.H, net.W, 64), H16), xa)
+    net.run(dc, na, 0, src=db)
+    assert np.array_equal(_download(rt, dc, s0.shape, np.float32), s0)
+    for d in (da, db, dc):
+        rt.free(d)
+    net.free()
+
+
 @pytest.mark.parametrize("elem", ELEMS)
 @pytest.mark.parametrize("shape", [(2, 32, 128, 64), (1, 20, 256, 32)])
 def test_gradients_match_oracle(rt, shape, elem):
@@ -478,17 +520,17 @@ def test_entry_points_reject_bad_shapes(rt):
         for i in range(4):
             net.act[i] = net.wpk[i] = net.bias[i] = p
         one = _f3((1.0, 1.0, 1.0))
-        rc = lib.lbm_sur_run(ctypes.byref(net), ctypes.c_void_p(p), one, one, one, 1, 1, 4, 16, rt.stream)
+        rc = lib.lbm_sur_run(ctypes.byref(net), None, ctypes.c_void_p(p), one, one, one, 1, 1, 4, 16, rt.stream)
         assert rc != 0 and b"xalt" in lib.lbm_last_error()
         net.xalt = p + 4096
-        rc = lib.lbm_sur_run(ctypes.byref(net), ctypes.c_void_p(p), one, one, one, 1, 1, 6, 16, rt.stream)
+        rc = lib.lbm_sur_run(ctypes.byref(net), None, ctypes.c_void_p(p), one, one, one, 1, 1, 6, 16, rt.stream)
         assert rc != 0 and b"multiple of 4x16" in lib.lbm_last_error()
         net.elem = 3
-        rc = lib.lbm_sur_run(ctypes.byref(net), ctypes.c_void_p(p), one, one, one, 1, 1, 4, 16, rt.stream)
+        rc = lib.lbm_sur_run(ctypes.byref(net), None, ctypes.c_void_p(p), one, one, one, 1, 1, 4, 16, rt.stream)
         assert rc != 0 and b"element type" in lib.lbm_last_error()
         net.elem = 0
         net.hidden = 48
-        rc = lib.lbm_sur_run(ctypes.byref(net), ctypes.c_void_p(p), one, one, one, 1, 1, 4, 16, rt.stream)
+        rc = lib.lbm_sur_run(ctypes.byref(net), None, ctypes.c_void_p(p), one, one, one, 1, 1, 4, 16, rt.stream)
         assert rc != 0 and b"hidden" in lib.lbm_last_error()
         rc = lib.lbm_sur_adam(ctypes.c_void_p(p), ctypes.c_void_p(p), ctypes.c_void_p(p), ctypes.c_void_p(p),
                               ctypes.c_void_p(p), ctypes.c_void_p(p), 4, 4, 4, 4, 1e-3, 0.9, 0.999, 1e-8, 0,

@@ -25,8 +25,8 @@ class _Net:
     def __init__(self, bufs, log):
         self.bufs, self.log = bufs, log
 
-    def run(self, ptr, norm_args, nsteps):
-        t, hist = self.bufs[ptr]
+    def run(self, ptr, norm_args, nsteps, src=None):
+        t, hist = self.bufs[ptr if src is None else src]
         self.bufs[ptr] = (t + nsteps, hist + "s" * nsteps)
         self.log.append(("net", nsteps))
 
