@@ -148,6 +148,16 @@ int lbm_sur_rel_l2_async(const float* d_a, const float* d_b, int64_t npix, doubl
 int lbm_sur_populations_to_state(const float* d_f, const lbm_layout* layout, const double* mc, int H, int W,
                                  float* d_state, void* stream);
 
+/* the read-back of a judge in one pass: the state of plain FP32 D2Q9 populations (as
+ * lbm_sur_populations_to_state) into d_state and, if d_ckpt is not NULL, d_ckpt; its
+ * network input (as lbm_sur_make_input, E = elem) into d_xin; and the drift sums
+ * against d_pred (as lbm_sur_rel_l2_async(d_pred, d_state), same accumulation order)
+ * into h_out[0..1] (pinned host memory) once the stream reaches this point.
+ * d_partial: 514 doubles.  H, W: multiples of 4. */
+int lbm_sur_judge_read(const float* d_f, const lbm_layout* layout, const double* mc, int H, int W,
+                       const float* d_pred, float* d_state, float* d_ckpt, const float* shift, const float* scale,
+                       void* d_xin, int elem, double* d_partial, double* h_out, void* stream);
+
 /* training noise: state += sigma[f] * N(0, 1) on the fields f = rho, ux, uy of every
  * fine pixel, from a counter-based generator keyed by seed (reproducible) */
 int lbm_sur_perturb(float* d_state, int64_t npix, const float* sigma, uint64_t seed, void* stream);

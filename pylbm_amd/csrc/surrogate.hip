@@ -62,6 +62,25 @@ constexpr int WG_PIX = 32;             // pixels per weight-gradient block itera
 constexpr int SUR_WGRAD_BLOCKS = 64;   // weight-gradient workgroups per tap
 
 __device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+// The 16-bit values every kernel stores are formed by these helpers.  The FP32 value
+// passes through an empty register constraint before the conversion, so (E)x always
+// rounds a rounded FP32 value: without it the backend may fold a multiply and the
+// conversion into one mixed-precision FMA (a single rounding) in some kernels and
+// not in others.  With the helpers the layered and fused paths, make_input and the
+// judge read-back store identical values for identical FP32 inputs.
+__device__ __forceinline__ float fp32_value(float x) {
+  asm("" : "+v"(x));
+  return x;
+}
+template <typename E>
+__device__ __forceinline__ E silu_e(float z) {
+  return (E)fp32_value(z * sigmoid(z));
+}
+template <typename E>
+__device__ __forceinline__ E input_e(float s, float shift, float inv_scale) {
+  return (E)fp32_value((s - shift) * inv_scale);
+}
 __device__ __forceinline__ int wrap(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
 // Block-wide copy of TOTAL 16-byte items (item e = thread + 256 i) from ld to st;
@@ -174,7 +193,7 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const E* __restrict__ in, 
       for (int q = 0; q < 4; ++q) {
         const float z = acc[mt][q] + bias[c0 + q];
         zb[q] = (E)z;
-        ab[q] = (E)(z * sigmoid(z));
+        ab[q] = silu_e<E>(z);
       }
       __builtin_memcpy((E*)out + p * COUTP + c0, &ab, 8);
       if (zout) __builtin_memcpy(zout + p * COUTP + c0, &zb, 8);
@@ -200,7 +219,7 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const E* __restrict__ in, 
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         s[q] += (acc[mt][q] + bias[c0 + q]) * ap.dscale[q];
-        xb[q] = (E)((s[q] - ap.shift[q]) * ap.inv_scale[q]);
+        xb[q] = input_e<E>(s[q], ap.shift[q], ap.inv_scale[q]);
       }
       xb[3] = (E)0.f;
       __builtin_memcpy(ap.state + fp * 4, &s, 16);
@@ -357,7 +376,7 @@ __device__ __forceinline__ void fused_layer(const E* __restrict__ src, E* __rest
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const float z = acc[u][mt][q] + bias[c0 + q];
-            ab[q] = (E)(z * sigmoid(z));
+            ab[q] = silu_e<E>(z);
           }
           __builtin_memcpy(dst + pc * CIPO + c0, &ab, 8);
         } else {
@@ -371,7 +390,7 @@ __device__ __forceinline__ void fused_layer(const E* __restrict__ src, E* __rest
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             sv[q] += (acc[u][mt][q] + bias[c0 + q]) * ap.dscale[q];
-            xb[q] = (E)((sv[q] - ap.shift[q]) * ap.inv_scale[q]);
+            xb[q] = input_e<E>(sv[q], ap.shift[q], ap.inv_scale[q]);
           }
           xb[3] = (E)0.f;
           __builtin_memcpy(ap.state + fp * 4, &sv, 16);
@@ -631,7 +650,7 @@ __global__ void __launch_bounds__(256) make_input_kernel(const float* __restrict
   __builtin_memcpy(&s, st + fp * 4, 16);
   half4 o;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) o[q] = (E)(q < 3 ? (s[q] - shift[q]) * inv_scale[q] : 0.f);
+  for (int q = 0; q < 4; ++q) o[q] = q < 3 ? input_e<E>(s[q], shift[q], inv_scale[q]) : (E)0.f;
   const int H = Hf / 4, W = Wf / 4;
   const int64_t cp = ((int64_t)n * H + fy / 4) * W + fx / 4;
   __builtin_memcpy(xin + cp * 64 + ((fy & 3) * 4 + (fx & 3)) * 4, &o, 8);
@@ -779,6 +798,82 @@ __global__ void __launch_bounds__(256) populations_to_state_kernel(const float* 
   }
   float4v s = {(float)m[0], (float)(m[1] / m[0]), (float)(m[2] / m[0]), 0.f};
   __builtin_memcpy(st + p * 4, &s, 16);
+}
+
+// The read-back of a judge in one pass over the fine pixels of a D2Q9 layout
+// (grid-stride over 256 x 256 threads, the accumulation order of rel_l2_kernel):
+// state from plain FP32 populations (populations_to_state_kernel) into st and, if
+// not NULL, ck; its network input (make_input_kernel) into xin; and the partial sums
+// of (pred - state)^2 and state^2 over the velocity fields (rel_l2_kernel with
+// a = pred, b = state).  A thread loads the populations of JR_U of its pixels before
+// it combines them, in the grid-stride order.
+constexpr int JR_U = 4;
+
+template <typename E>
+__global__ void __launch_bounds__(256) judge_read_kernel(const float* __restrict__ f, lbm_layout L, ConservedRows mc,
+                                                         int H, int W, const float* __restrict__ pred,
+                                                         float* __restrict__ st, float* __restrict__ ck,
+                                                         float4v shift, float4v inv_scale, E* __restrict__ xin,
+                                                         double* __restrict__ partial) {
+  typedef typename Elem<E>::v4 half4;
+  constexpr int Q = 9;
+  __shared__ double s0[256], s1[256];
+  double d = 0.0, nn = 0.0;
+  const int64_t npix = (int64_t)H * W;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int Wc = W / 4;
+  for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p0 < npix; p0 += JR_U * stride) {
+    float v[JR_U][Q];
+    int64_t cell[JR_U];
+#pragma unroll
+    for (int u = 0; u < JR_U; ++u) {
+      const int64_t p = p0 + u * stride;
+      const int64_t pc = p < npix ? p : p0;  // past the end: a valid pixel, not used
+      cell[u] = ::lbm::cell_offset(L, 0, (int)(pc / W) + L.halo[1], (int)(pc % W) + L.halo[2]);
+#pragma unroll
+      for (int k = 0; k < Q; ++k) v[u][k] = f[k * L.pop_stride + cell[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < JR_U; ++u) {
+      const int64_t p = p0 + u * stride;
+      if (p >= npix) break;
+      double m[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < Q; ++k) {
+        const double x = (double)v[u][k];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i] += mc.c[i][k] * x;
+      }
+      const float4v sv = {(float)m[0], (float)(m[1] / m[0]), (float)(m[2] / m[0]), 0.f};
+      __builtin_memcpy(st + p * 4, &sv, 16);
+      if (ck) __builtin_memcpy(ck + p * 4, &sv, 16);
+      for (int q = 1; q < 3; ++q) {
+        const double x = pred[p * 4 + q], y = sv[q];
+        d += (x - y) * (x - y);
+        nn += y * y;
+      }
+      half4 o;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o[q] = q < 3 ? input_e<E>(sv[q], shift[q], inv_scale[q]) : (E)0.f;
+      const int yy = (int)(p / W), xx = (int)(p % W);
+      const int64_t cp = (int64_t)(yy / 4) * Wc + xx / 4;
+      __builtin_memcpy(xin + cp * 64 + ((yy & 3) * 4 + (xx & 3)) * 4, &o, 8);
+    }
+  }
+  s0[threadIdx.x] = d;
+  s1[threadIdx.x] = nn;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      s0[threadIdx.x] += s0[threadIdx.x + off];
+      s1[threadIdx.x] += s1[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s0[0];
+    partial[2 * blockIdx.x + 1] = s1[0];
+  }
 }
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -1070,6 +1165,38 @@ int lbm_sur_populations_to_state(const float* d_f, const lbm_layout* L, const do
   populations_to_state_kernel<<<blocks_for((int64_t)H * W), 256, 0, (hipStream_t)stream>>>(d_f, *L, rows, H, W,
                                                                                           d_state);
   LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int lbm_sur_judge_read(const float* d_f, const lbm_layout* L, const double* mc, int H, int W, const float* d_pred,
+                       float* d_state, float* d_ckpt, const float* shift, const float* scale, void* d_xin, int elem,
+                       double* d_partial, double* h_out, void* stream) {
+  int rc = check_layout(L);
+  if (rc) return rc;
+  LBM_REQUIRE(L->storage == LBM_F32, "judge_read: plain FP32 populations only (storage %d)", L->storage);
+  LBM_REQUIRE(L->ns == 9, "judge_read: D2Q9 populations only (ns %d)", (int)L->ns);
+  LBM_REQUIRE(L->dim == 2 && L->n[1] - 2 * L->halo[1] == H && L->n[2] - 2 * L->halo[2] == W,
+              "judge_read: layout is not a 2D %dx%d interior", H, W);
+  LBM_REQUIRE(H % 4 == 0 && W % 4 == 0, "judge_read: the grid must be a multiple of 4");
+  LBM_REQUIRE(d_pred != nullptr && d_state != nullptr && d_xin != nullptr && d_partial != nullptr,
+              "judge_read: NULL buffer");
+  ConservedRows rows;
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < LBM_MAX_Q; ++k) rows.c[i][k] = k < L->ns ? mc[i * L->ns + k] : 0.0;
+  const float4v sh = {shift[0], shift[1], shift[2], 0.f};
+  const float4v is = {1.f / scale[0], 1.f / scale[1], 1.f / scale[2], 0.f};
+  hipStream_t s = (hipStream_t)stream;
+  rc = with_elem(elem, "judge_read", [&](auto tag) -> int {
+    using E = typename decltype(tag)::type;
+    judge_read_kernel<E><<<256, 256, 0, s>>>(d_f, *L, rows, H, W, d_pred, d_state, d_ckpt, sh, is, (E*)d_xin,
+                                             d_partial);
+    return 0;
+  });
+  if (rc) return rc;
+  LBM_CHECK_HIP(hipGetLastError());
+  rel_l2_final_kernel<<<1, 256, 0, s>>>(d_partial, d_partial + 512);
+  LBM_CHECK_HIP(hipGetLastError());
+  LBM_CHECK_HIP(hipMemcpyAsync(h_out, d_partial + 512, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   return 0;
 }
 

@@ -121,6 +121,8 @@ SURROGATE_SIGNATURES = {
     "lbm_sur_perturb": ([_P, _I64, _FP, ctypes.c_uint64, _P], _I),
     "lbm_sur_rel_l2_async": ([_P, _P, _I64, _P, _P, _P], _I),
     "lbm_sur_populations_to_state": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(_D), _I, _I, _P, _P], _I),
+    "lbm_sur_judge_read": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(_D), _I, _I, _P, _P, _P, _FP, _FP, _P, _I, _P,
+                            _P, _P], _I),
 }
 
 GEN_SIGNATURES = {

@@ -584,6 +584,26 @@ class SurrogateRun:
                                                     self.rt.stream), "lbm_sur_rel_l2_async")
         self.rt.record(self.events[slot])
 
+    def _judge_read(self, slot, ckpt):
+        """After the lattice ran a judged interval: its state into self.state and ckpt,
+        the network input of that state into xin, and the drift of pred against it
+        into pinned slot `slot` (one kernel for plain FP32 populations)."""
+        lat = self.lat
+        if not lat._direct:
+            lat.read_state(self.state)
+            self._drift_async(slot)
+            self.rt.d2d(ckpt, self.state, lat.npix * 16)
+            self._make_input(self.state)
+            return
+        F = lat.sim.container.F
+        self.rt.check(self.lib.lbm_sur_judge_read(ctypes.c_void_p(F.ptr), ctypes.byref(F.layout), lat._mc, lat.H,
+                                                  lat.W, ctypes.c_void_p(self.pred), ctypes.c_void_p(self.state),
+                                                  ctypes.c_void_p(ckpt), self._shift, self._scale,
+                                                  ctypes.c_void_p(self.net.xin), self.net.elem,
+                                                  ctypes.c_void_p(self.lpart), ctypes.c_void_p(self.hdrift + 16 * slot),
+                                                  self.rt.stream), "lbm_sur_judge_read")
+        self.rt.record(self.events[slot])
+
     def _drift_wait(self, slot):
         self.rt.check(self.lib.lbm_event_sync(self.events[slot]), "lbm_event_sync")
         d, n = (ctypes.c_double * 2).from_address(self.hdrift + 16 * slot)
@@ -736,9 +756,8 @@ class SurrogateRun:
             self.net.run(self.pred, norm_args, 1, src=self.state)
             lat.restart(self.state)
             lat.run(K)
-            lat.read_state(self.state)
-            self._drift_async(j % 2)
-            self.rt.d2d(self.ckpt[j % 3], self.state, nbytes)
+            # lattice state -> state and checkpoint j % 3, its network input, drift
+            self._judge_read(j % 2, self.ckpt[j % 3])
             i += 1
             if pending is not None:
                 if not check(pending):
@@ -747,11 +766,13 @@ class SurrogateRun:
                 confirmed, confirmed_at = pending[0] % 3, pending[1]
             pending = (j, i)
             j += 1
-            self._make_input(self.state)
         if pending is not None and not check(pending):
             fall_back(pending[1] - 1)
             return
-        lat.restart(self.state)
+        if nintervals % je:
+            # the network did the last intervals: the lattice takes its state (after a
+            # judge the lattice already holds the state, with its own populations)
+            lat.restart(self.state)
 
 
 def train_surrogate(lattice, ntraj=200, steps=2000, K=16, pool=8, batch=8, steps_per_traj=100, lr=2e-3,

@@ -431,6 +431,46 @@ def test_rel_l2_kernel(rt, lattice):
     net.free()
 
 
+@pytest.mark.parametrize("elem", ELEMS)
+def test_judge_read_matches_separate_calls(rt, lattice, elem):
+    """lbm_sur_judge_read (one pass) = read_state + rel_l2_async + checkpoint copy +
+    make_input, bit for bit, drift sums included."""
+    from pylbm_amd.surrogate import Normalization, SurrogateNet, SurrogateRun
+    lat = lattice
+    net = SurrogateNet(rt, lat.H, lat.W, batch=1, hidden=32, train=False, elem=elem)
+    run = SurrogateRun(net, lat, Normalization((1.0, 0.0, 0.0), (1e-3, 0.05, 0.05), (1e-4, 1e-3, 1e-3)))
+    s = lat.initial_state(3, kmin=1.0, kmax=4.0)
+    rt.h2d(run.state, s)
+    lat.restart(run.state)
+    lat.run(16)
+    rng = np.random.default_rng(4)
+    pred = s + np.float32(1e-3) * rng.normal(size=s.shape).astype(np.float32)
+    pred[..., 3] = 0
+    rt.h2d(run.pred, pred)
+    xshape = (lat.H // 4, lat.W // 4, 64)
+    lat.read_state(run.state)
+    run._drift_async(0)
+    rt.d2d(run.ckpt[0], run.state, lat.npix * 16)
+    run._make_input(run.state)
+    d0 = run._drift_wait(0)
+    sums0 = tuple((ctypes.c_double * 2).from_address(run.hdrift))
+    st0 = _download(rt, run.state, s.shape, np.float32)
+    x0 = _download(rt, net.xin, xshape, _store(elem))
+    assert np.array_equal(_download(rt, run.ckpt[0], s.shape, np.float32), st0)
+    for buf in (run.state, run.ckpt[1]):
+        rt.h2d(buf, np.full_like(s, np.nan))
+    rt.h2d(net.xin, np.zeros(xshape, _store(elem)))
+    run._judge_read(1, run.ckpt[1])
+    d1 = run._drift_wait(1)
+    assert tuple((ctypes.c_double * 2).from_address(run.hdrift + 16)) == sums0
+    assert d1 == d0 and 0 < d0 < 1
+    assert np.array_equal(_download(rt, run.state, s.shape, np.float32), st0)
+    assert np.array_equal(_download(rt, run.ckpt[1], s.shape, np.float32), st0)
+    assert np.array_equal(_download(rt, net.xin, xshape, _store(elem)), x0)
+    run.free()
+    net.free()
+
+
 def test_surrogate_gain_over_plain_lattice(rt, lattice):
     """What the trained network gains over the plain lattice run.
 

@@ -75,7 +75,13 @@ def _run(nintervals, je, bad=()):
         k = max(i for i, (s, _) in enumerate(judged) if s == slot)
         return 1.0 if k in bad else 1e-3
 
+    def judge_read(slot, ckpt):
+        run.lat.read_state("state")
+        drift_async(slot)
+        run.rt.d2d(ckpt, "state", 0)
+
     run._drift_async = drift_async
+    run._judge_read = judge_read
     run._drift_wait = drift_wait
     run._make_input = lambda ptr: None
     bufs["state"] = (0, "")
@@ -91,6 +97,8 @@ def test_all_judges_pass(nint, je):
     assert len(run.judges) == nint // je
     expect = "".join("L" if (i + 1) % je == 0 else "s" for i in range(nint))
     assert hist == expect
+    # the lattice ends holding the state (restarted from it, or its own judged result)
+    assert run.lat.cur == bufs["state"]
 
 
 @pytest.mark.parametrize("nint,je,bad", [(64, 8, 0), (64, 8, 1), (64, 8, 3), (64, 8, 7), (20, 4, 2), (5, 1, 0)])
@@ -105,6 +113,7 @@ def test_failed_judge_falls_back_to_last_passed_checkpoint(nint, je, bad):
     prefix = "".join("L" if (i + 1) % je == 0 else "s" for i in range(start))
     assert hist == prefix + "L" * (nint - start)
     assert len(run.judges) == bad + 1
+    assert run.lat.cur == bufs["state"]
 
 
 # ------------------------------------------------------------ overlapped schedule
