@@ -210,9 +210,21 @@ typedef struct lbm_link_set {
   double sign;
   int64_t n;
   int32_t two_phase;
-  int32_t _pad;
+  /* 1: no slot this list loads or stores is stored by the lists of the group it
+   * joins (the previous list and those it joined), and it loads no slot those
+   * lists store: it may run in the same launch as them (set by the host) */
+  int32_t merge_prev;
   void* d_work;
 } lbm_link_set;
+
+/*
+ * The link lists of one time step in order (pylbm/simulation.py:385-390), in as
+ * few launches as their merge_prev flags allow: a group of up to 8 consecutive
+ * one-phase lists joined by merge_prev runs as one kernel, with each link
+ * computed as lbm_links_apply computes it.
+ */
+int lbm_links_apply_sets(void* d_f, const lbm_layout* layout, const lbm_link_set* sets, int nsets,
+                         void* stream);
 
 int lbm_gen_run(void* d_f, void* d_fnew, const lbm_layout* layout, const lbm_velocities* vel,
                 lbm_params* params, const lbm_link_set* links, int nmeth, int64_t nsteps,

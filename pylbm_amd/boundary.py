@@ -172,6 +172,74 @@ def _resolve_sequential(store, load, sign, rhs):
     return store[keep], out_l[keep], out_a[keep], out_r[keep]
 
 
+MAX_LINK_GROUP = 8  # lists per launch of lbm_links_apply_sets
+
+
+def merge_flags(lists):
+    """merge_prev flags of the link lists of one step, in application order.
+
+    lists: [(store, load, two_phase)] with flat element indices.  List j joins
+    the group of the lists before it (flag 1) when it is one-phase, the group
+    is one-phase and has fewer than MAX_LINK_GROUP lists, no list of the group
+    stores a slot that j loads or stores, and j stores no slot the group loads.
+    Then every link of the group reads the value the sequential order
+    (pylbm/simulation.py:385-390) gives it and every slot receives its one
+    store, so the group may run as one launch.  The grouping is the one
+    lbm_links_apply_sets applies."""
+    flags = []
+    gst = gld = None
+    gsize = 0
+    gtwo = True
+    for st, ld, two in lists:
+        st = np.asarray(st)
+        ld = np.asarray(ld)
+        ok = (not two and not gtwo and 0 < gsize < MAX_LINK_GROUP
+              and not np.isin(st, gst).any() and not np.isin(ld, gst).any() and not np.isin(st, gld).any())
+        if ok:
+            gst = np.union1d(gst, st)
+            gld = np.union1d(gld, ld)
+            gsize += 1
+        else:
+            gst, gld, gsize, gtwo = np.unique(st), np.unique(ld), 1, bool(two)
+        flags.append(1 if ok else 0)
+    return flags
+
+
+def host_lists(method, layout, wshift, flat=None):
+    """The index lists one link method runs with, as DeviceLinks uploads them:
+    (store, load, sign, rhs, conflict, order) with flat element indices of layout.
+    flat=(store, load): a plain copy list f[store] = f[load] (method is only a name).
+    conflict: the list loads a slot it stores or stores one slot twice; its
+    sequential chains are then folded into load/sign/rhs (run two-phase).
+    order: the permutation that sorted a non-conflicting list by store (or None)."""
+    if flat is None:
+        rows_s = method._store_rows()
+        rows_l = method.iload[0].T if method._fixed else method.iload[0]
+        st = _flat(rows_s, layout)
+        ld = _flat(rows_l, layout)
+        sign = np.full(st.size, method.kind.sign)
+        rhs = method.rhs.copy() if method.kind.rhs_sign is not None else np.zeros(st.size)
+    else:
+        st, ld = (np.asarray(a, dtype=np.int64) for a in flat)
+        sign = np.ones(st.size)
+        rhs = np.zeros(st.size)
+    conflict = bool((np.intersect1d(st, ld).size > 0) or (np.unique(st).size != st.size))
+    order = None
+    if conflict:
+        st, ld, sign, rhs = _resolve_sequential(st, ld, sign, rhs)
+    elif st.size > 1:
+        # no slot is both stored and loaded and every store is unique, so the list
+        # may run in any order: sort by store address, which puts the links of one
+        # (o, m) row (all populations, interleaved layout) next to each other and
+        # lets the lanes of a wave share cache lines
+        order = np.argsort(st, kind="stable")
+        st, ld, sign, rhs = st[order], ld[order], sign[order], rhs[order]
+    if layout.storage in DEVIATION_FORMATS:
+        # deviations: d_s = a d_l + (r + a w_l - w_s)
+        rhs = rhs + sign * wshift[_population(ld, layout)] - wshift[_population(st, layout)]
+    return st, ld, sign, rhs, conflict, order
+
+
 class DeviceLinks:
     """Device arrays of one method for a given population layout."""
 
@@ -181,37 +249,11 @@ class DeviceLinks:
         self.rt = rt
         self.method = method
         self.layout = layout
-        if flat is None:
-            rows_s = method._store_rows()
-            rows_l = method.iload[0].T if method._fixed else method.iload[0]
-            st = _flat(rows_s, layout)
-            ld = _flat(rows_l, layout)
-            self.n = st.size
-            sign = np.full(self.n, method.kind.sign)
-            rhs = method.rhs.copy() if method.kind.rhs_sign is not None else np.zeros(self.n)
-        else:
-            st, ld = (np.asarray(a, dtype=np.int64) for a in flat)
-            self.n = st.size
-            sign = np.ones(self.n)
-            rhs = np.zeros(self.n)
-        self.conflict = (np.intersect1d(st, ld).size > 0) or (np.unique(st).size != st.size)
-        self.order = None
-        if self.conflict:
-            st, ld, sign, rhs = _resolve_sequential(st, ld, sign, rhs)
-            self.n = st.size
-        elif self.n > 1:
-            # no slot is both stored and loaded and every store is unique, so the list
-            # may run in any order: sort by store address, which puts the links of one
-            # (o, m) row (all populations, interleaved layout) next to each other and
-            # lets the lanes of a wave share cache lines
-            self.order = np.argsort(st, kind="stable")
-            st, ld, sign, rhs = st[self.order], ld[self.order], sign[self.order], rhs[self.order]
-        kstore = _population(st, layout)
-        kload = _population(ld, layout)
-        if layout.storage in DEVIATION_FORMATS:
-            # deviations: d_s = a d_l + (r + a w_l - w_s)
-            rhs = rhs + sign * wshift[kload] - wshift[kstore]
+        st, ld, sign, rhs, self.conflict, self.order = host_lists(method, layout, wshift, flat)
+        self.n = st.size
         self.rhs_host = rhs
+        # host copies of the final index lists, for merge_flags (dropped after use)
+        self.h_store, self.h_load = st, ld
         ctype = np.float64 if layout.storage == LBM_F64 else np.float32
         self._ptrs = []
         self.d_store = self._up(st.astype(np.uint32))
@@ -254,6 +296,7 @@ class DeviceLinks:
         s.sign = self.sign
         s.n = self.n
         s.two_phase = 1 if self.conflict else 0
+        s.merge_prev = 0
         s.d_work = self.d_work
         return s
 

@@ -196,6 +196,64 @@ static int links_launch(void* d_f, const uint32_t* st, const uint32_t* ld, const
   return 0;
 }
 
+// Several independent link lists (no list reads or writes a slot another one writes)
+// in one launch: thread t belongs to the list j with end[j - 1] <= t < end[j] and
+// applies that list's link t - end[j - 1] exactly as links_kernel does.
+constexpr int LBM_MAX_LINK_GROUP = 8;
+
+template <typename C>
+struct LinkGroup {
+  const uint32_t* store[LBM_MAX_LINK_GROUP];
+  const uint32_t* load[LBM_MAX_LINK_GROUP];
+  const C* rhs[LBM_MAX_LINK_GROUP];
+  const int8_t* sgn[LBM_MAX_LINK_GROUP];
+  float sign[LBM_MAX_LINK_GROUP];
+  int64_t end[LBM_MAX_LINK_GROUP];
+  int count;
+};
+
+template <typename S>
+__global__ void __launch_bounds__(256) links_group_kernel(S* __restrict__ f,
+                                                          const LinkGroup<typename Codec<S>::C> G) {
+  using C = typename Codec<S>::C;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.end[G.count - 1]) return;
+  int j = 0;
+  while (t >= G.end[j]) ++j;
+  const int64_t i = t - (j ? G.end[j - 1] : 0);
+  C v = Codec<S>::load(f, G.load[j][i], (C)0);
+  const C a = G.sgn[j] ? (C)G.sgn[j][i] : (C)G.sign[j];
+  C r = a * v;
+  if (G.rhs[j]) r += G.rhs[j][i];
+  Codec<S>::store(f, G.store[j][i], r, (C)0);
+}
+
+template <typename S>
+static int links_group_launch(void* d_f, const lbm_link_set* sets, int nsets, hipStream_t s) {
+  using C = typename Codec<S>::C;
+  LinkGroup<C> G{};
+  int64_t total = 0;
+  G.count = nsets;
+  for (int j = 0; j < nsets; ++j) {
+    G.store[j] = sets[j].d_store;
+    G.load[j] = sets[j].d_load;
+    G.rhs[j] = (const C*)sets[j].d_rhs;
+    G.sgn[j] = sets[j].d_sign;
+    G.sign[j] = (float)sets[j].sign;
+    total += sets[j].n;
+    G.end[j] = total;
+  }
+  if (total == 0) return 0;
+  links_group_kernel<S><<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>((S*)d_f, G);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+static int links_group_dispatch(void* d_f, const lbm_layout* layout, const lbm_link_set* sets, int nsets,
+                                hipStream_t s) {
+  LBM_DISPATCH_STORAGE(layout->storage, links_group_launch, d_f, sets, nsets, s);
+}
+
 // ---------------------------------------------------- compensated sums
 // Neumaier accumulation per thread, TwoSum-based pairwise combination of
 // (sum, compensation) pairs across the block and then across blocks.
@@ -481,6 +539,34 @@ int lbm_links_apply(void* d_f, const lbm_layout* layout, const uint32_t* d_store
   hipStream_t s = (hipStream_t)stream;
   LBM_DISPATCH_STORAGE(layout->storage, links_launch, d_f, d_store, d_load, d_rhs, d_sign, sign, n,
                        two_phase, d_work, s);
+}
+
+int lbm_links_apply_sets(void* d_f, const lbm_layout* layout, const lbm_link_set* sets, int nsets,
+                         void* stream) {
+  int rc = check_layout(layout);
+  if (rc) return rc;
+  LBM_REQUIRE(nsets >= 0 && (nsets == 0 || sets != nullptr), "link sets: NULL array");
+  hipStream_t s = (hipStream_t)stream;
+  int j = 0;
+  while (j < nsets) {
+    // a group: set j and the following sets marked merge_prev (at most 8, none two-phase)
+    int k = j + 1;
+    if (!sets[j].two_phase) {
+      while (k < nsets && k - j < LBM_MAX_LINK_GROUP && sets[k].merge_prev && !sets[k].two_phase) ++k;
+    }
+    if (k - j == 1) {
+      const lbm_link_set& L = sets[j];
+      rc = lbm_links_apply(d_f, layout, L.d_store, L.d_load, L.d_rhs, L.d_sign, L.sign, L.n, L.two_phase,
+                           L.d_work, stream);
+    } else {
+      for (int q = j; q < k; ++q)
+        LBM_REQUIRE(sets[q].n == 0 || (sets[q].d_store && sets[q].d_load), "store/load index arrays are NULL");
+      rc = links_group_dispatch(d_f, layout, sets + j, k - j, s);
+    }
+    if (rc) return rc;
+    j = k;
+  }
+  return 0;
 }
 
 int lbm_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream) {

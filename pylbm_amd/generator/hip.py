@@ -648,12 +648,8 @@ int lbm_gen_run(void* d_f, void* d_fnew, const lbm_layout* layout, const lbm_vel
     if (ghost == 1) rc = lbm_ghost_fill(a, layout, vel, stream);
     else if (ghost == 2) rc = lbm_ghost_fill_all(a, layout, vel, stream);
     if (rc) return rc;
-    for (int j = 0; j < nmeth; ++j) {
-      const lbm_link_set& L = links[j];
-      rc = lbm_links_apply(a, layout, L.d_store, L.d_load, L.d_rhs, L.d_sign, L.sign, L.n,
-                           L.two_phase, L.d_work, stream);
-      if (rc) return rc;
-    }
+    if (nmeth > 0) rc = lbm_links_apply_sets(a, layout, links, nmeth, stream);
+    if (rc) return rc;
     rc = lbm_gen_step(a, b, layout, params, periodic_fold, stream);
     if (rc) return rc;
     void* tmp = a;

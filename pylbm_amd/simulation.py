@@ -26,7 +26,7 @@ import numpy as np
 import sympy as sp
 
 from . import runtime as rtm
-from .boundary import Boundary, DeviceLinks, unwritten_ghost_reads
+from .boundary import Boundary, DeviceLinks, merge_flags, unwritten_ghost_reads
 from .domain import Domain
 from .generator.build import load_module
 from .monitoring import Monitor
@@ -151,6 +151,9 @@ class Simulation:
         # slots any pull or Neumann-type list reads come from a host-computed copy list
         # (boundary.unwritten_ghost_reads); the full wrap (2) stays available
         self._ghost_mode = 1
+        # consecutive independent link lists share one launch in run() (False: one
+        # launch per list; the results are the same)
+        self.merge_links = True
         self.init_type = dico.get("inittype", "moments")
         self.init_data = dico.get("init", None)
         self._need_init = True
@@ -222,6 +225,12 @@ class Simulation:
                                                       self.wshift, flat=(st, ld)))
         self._link_sets = (rtm.LbmLinkSet * max(1, len(self._dev_links)))(
             *[d.link_set() for d in self._dev_links])
+        # independent consecutive lists run in one launch (bit-identical results)
+        flags = merge_flags([(d.h_store, d.h_load, d.conflict) for d in self._dev_links])
+        for ls, flag in zip(self._link_sets, flags):
+            ls.merge_prev = flag if self.merge_links else 0
+        for d in self._dev_links:
+            d.h_store = d.h_load = None
         self._need_init = False
 
     def initialization(self):

@@ -127,6 +127,8 @@ def test_flat_indices_both_arrangements():
 class _HostLayout:
     """Layout whose flat indices address an oracle array (ns, *shape_halo) directly."""
 
+    storage = 0  # LBM_F64: no deviation shift
+
     def __init__(self, ns, shape):
         n = [1] * (3 - len(shape)) + list(shape)
         self.ns, self.n = ns, n
@@ -208,3 +210,72 @@ def test_restricted_ghost_fill_equals_full_wrap(name):
     assert np.array_equal(a[inner], b[inner])
     if name == "spheres":
         assert ncopies > 0
+
+
+def test_merge_flags_synthetic():
+    """Dependencies that must split a group, the group size cap, two-phase lists."""
+    from pylbm_amd.boundary import MAX_LINK_GROUP, merge_flags
+    a = (np.array([0, 1]), np.array([10, 11]), False)
+    assert merge_flags([a, (np.array([2]), np.array([12]), False)]) == [0, 1]          # independent
+    assert merge_flags([a, (np.array([3]), np.array([0]), False)]) == [0, 0]           # loads a's store
+    assert merge_flags([a, (np.array([11]), np.array([13]), False)]) == [0, 0]         # stores a's load
+    assert merge_flags([a, (np.array([1]), np.array([14]), False)]) == [0, 0]          # stores a's store
+    two = (np.array([20]), np.array([21]), True)
+    assert merge_flags([a, two]) == [0, 0] and merge_flags([two, a]) == [0, 0]
+    # a dependency on an earlier member of the group, not only the previous list
+    b = (np.array([2]), np.array([12]), False)
+    assert merge_flags([a, b, (np.array([4]), np.array([1]), False)]) == [0, 1, 0]
+    many = [(np.array([100 + i]), np.array([200 + i]), False) for i in range(MAX_LINK_GROUP + 2)]
+    assert merge_flags(many) == [0] + [1] * (MAX_LINK_GROUP - 1) + [0, 1]
+
+
+def _apply(F, lst):
+    st, ld, sign, rhs = lst[:4]
+    flat = F.reshape(-1)
+    flat[st] = sign * flat[ld] + rhs  # gather before scatter: one-phase and folded lists alike
+
+
+@pytest.mark.parametrize("name", ["spheres", "lid", "d1q3_walls", "karman", "vectorial", "neumann_periodic"])
+def test_merged_link_groups_equal_sequential(name):
+    """The step's lists (ghost copy list first) applied one after another equal the
+    groups merge_flags forms, each applied as one gather-then-scatter, bit for bit."""
+    from oracle.lbm import Simulation as O
+    from pylbm_amd.boundary import host_lists, merge_flags, unwritten_ghost_reads
+    d = {"spheres": lambda: cases.sphere_packing_d3q19(20, 14, 12, seed=3, solid_fraction=0.25),
+         "lid": lambda: cases.lid_cavity_d2q9(12),
+         "d1q3_walls": lambda: cases.d1q3_advection(10, walls=True),
+         "karman": lambda: cases.karman_d2q9(40, 20, radius=4.0, center=(10.0, 10.0)),
+         "vectorial": lambda: cases.d2q5q9_vectorial(12, walls=True),
+         "neumann_periodic": _neumann_periodic}[name]()
+    o = O(d)
+    dom, pm = _product(d)
+    for m in pm:
+        m.fix_iload()
+    lay = _HostLayout(o.ns, o.F.shape[1:])
+    lists = []
+    st, ld = unwritten_ghost_reads(dom, dom.stencil, pm, lay)
+    if st.size:
+        lists.append(host_lists("ghost_copy", lay, None, flat=(st, ld)))
+    rng = np.random.default_rng(1)
+    for m in pm:
+        m.rhs = rng.normal(size=m.rhs.shape)
+        lists.append(host_lists(m, lay, None))
+    flags = merge_flags([(x[0], x[1], x[4]) for x in lists])
+    F0 = rng.normal(size=o.F.shape)
+    seq = F0.copy()
+    for lst in lists:
+        _apply(seq, lst)
+    grp = F0.copy()
+    j = 0
+    while j < len(lists):
+        k = j + 1
+        while k < len(lists) and flags[k]:
+            k += 1
+        snap = grp.reshape(-1).copy()
+        vals = [lst[2] * snap[lst[1]] + lst[3] for lst in lists[j:k]]
+        for lst, v in zip(lists[j:k], vals):
+            grp.reshape(-1)[lst[0]] = v
+        j = k
+    assert np.array_equal(seq, grp)
+    if name in ("karman", "vectorial"):
+        assert sum(flags) > 0, flags
