@@ -129,7 +129,8 @@ def _cpu_baseline(target_s=8.0):
 
 
 def _link_timing(sim, reps=20):
-    """Per-launch time and achieved bytes of each link list of sim, applied alone."""
+    """Per-launch time and achieved bytes of each link list of sim, applied alone (a list
+    fused into the step, fused_into_step, is timed as the launch it replaces)."""
     rt = sim.rt
     out = []
     F = sim.container.F
@@ -147,7 +148,8 @@ def _link_timing(sim, reps=20):
         rhs_bytes = es if dl.d_rhs is not None else 0
         per_link = 8 + 2 * es + rhs_bytes + (1 if dl.d_sign is not None else 0)
         name = getattr(dl.method, "name", None) or str(getattr(dl.method, "kind", dl.method))
-        out.append({"method": str(name), "links": int(dl.n), "us_per_launch": round(1e3 * ms, 2),
+        out.append({"method": str(name), "links": int(dl.n), "fused_into_step": bool(getattr(dl, "fused", False)),
+                    "us_per_launch": round(1e3 * ms, 2),
                     "Glinks_per_s": round(dl.n / (ms * 1e-3) / 1e9, 3),
                     "GBps_alg": round(dl.n * per_link / (ms * 1e-3) / 1e9, 1), "bytes_per_link": per_link})
     return out

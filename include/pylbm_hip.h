@@ -170,6 +170,13 @@ typedef struct lbm_params {
   double cref[LBM_MAX_Q];   /* reference conserved moments (rest state) of the deviation formats */
   double dt;
   double t;
+  /* NULL, or one word per interior cell (outer, mid, inner order): bit k set means
+   * the pull of population k at this cell reads the cell's own population ksym
+   * instead of f[k, x - v_k].  That equals a zero-value bounce-back link
+   * f[k, x - v_k] = f[ksym, x] (pylbm/boundary.py:411-427) applied before the
+   * step, when no later link list of the step reads or writes its slots (the host
+   * checks this and then leaves the list out of the step's link sets). */
+  const uint32_t* bbmask;
 } lbm_params;
 
 /* information on the generated module (ns, nconsm, dim) */

@@ -363,6 +363,7 @@ _TEMPLATE = r"""// Generated by pylbm_amd.generator.hip for scheme %(hash)s -- d
 using namespace lbm;
 
 #define NS %(ns)d
+#define BBSUB_OK %(bbsub_ok)d  // the step can substitute fused bounce-back pulls
 #define NCONSM %(nc)d
 #define NCREF (NCONSM > 0 ? NCONSM : 1)
 #define DIM %(dim)d
@@ -376,6 +377,7 @@ struct StepArgs {
   int32_t nin[3];        // interior extents
   int32_t halo[3];
   int32_t n1, rs, lead, _pad;      // rs: elements between consecutive rows
+  const uint32_t* bbmask;          // NULL or fused bounce-back directions per interior cell
   C omega[NS], one_minus[NS], wshift[NS];
   C cref[NCREF];
   C half_dt, t;
@@ -429,6 +431,7 @@ fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename 
   const int o = oo + a.halo[0], m = mm + a.halo[1], ii = i + a.halo[2];
   const int cell = (o * a.n1 + m) * a.rs + a.lead + ii;
 %(loads)s
+%(bbsub)s
 %(gdecl)s
 #pragma unroll
   for (int j = 0; j < VEC; ++j) {
@@ -516,6 +519,8 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   for (int c = 0; c < NCREF; ++c) a->cref[c] = p ? (C)p->cref[c] : (C)0;
   a->half_dt = p ? (C)(0.5 * p->dt) : (C)0;
   a->t = p ? (C)p->t : (C)0;
+  a->bbmask = p ? p->bbmask : nullptr;
+  LBM_REQUIRE(a->bbmask == nullptr || BBSUB_OK, "this stencil has no symmetric pairs: no fused bounce-back");
   return 0;
 }
 
@@ -686,6 +691,22 @@ def generate_source(spec: KernelSpec) -> str:
     mvars = ["m%d" % i for i in range(ns)]
 
     # ---------------------------------------------------------- fused step
+    # fused zero-value bounce-back: where bit k of the cell's mask word is set, the
+    # pulled value of population k is the cell's own population ksym (the value
+    # the link f[k, x - v_k] = f[ksym, x] would have stored)
+    bbsub = []
+    if ksym is not None:
+        bbsub = ["if constexpr (!FOLD) {",
+                 "  if (a.bbmask) {",
+                 "    const uint32_t* bm = a.bbmask + ((int64_t)oo * a.nin[1] + mm) * a.nin[2] + i;",
+                 "#pragma unroll",
+                 "    for (int j = 0; j < VEC; ++j) {",
+                 "      const uint32_t bits = bm[j];",
+                 "      if (bits) {"]
+        for k in range(ns):
+            if ksym[k] != k:
+                bbsub.append("        if (bits & %du) L%d[j] = f[a.dst_off[%d] + cell + j];" % (1 << k, k, ksym[k]))
+        bbsub += ["      }", "    }", "  }", "}"]
     loads = ["S %s;" % ", ".join("L%d[VEC]" % k for k in range(ns))]
     loads.append("if (!FOLD) {")
     for k in range(ns):
@@ -766,7 +787,8 @@ def generate_source(spec: KernelSpec) -> str:
     return _TEMPLATE % dict(
         hash=spec_hash(spec), ns=ns, nc=spec.nconsm, dim=dim, paired=bool(sig_f2m),
         vel=vel_txt,
-        loads=_indent(loads), gdecl=_indent(gdecl), decode=_indent(decode, 4), vstores=_indent(vstores),
+        loads=_indent(loads), bbsub=_indent(bbsub), bbsub_ok=int(ksym is not None),
+        gdecl=_indent(gdecl), decode=_indent(decode, 4), vstores=_indent(vstores),
         f2m=_indent(f2m_block, 4), coll=_indent(coll, 4), coll_dev=_indent(coll_dev, 4),
         m2f=_indent(m2f_block, 4), m2f_dev=_indent(m2f_dev_block, 4), stores_dev=_indent(stores_dev, 4),
         stores=_indent(stores, 4),

@@ -154,6 +154,9 @@ class Simulation:
         # consecutive independent link lists share one launch in run() (False: one
         # launch per list; the results are the same)
         self.merge_links = True
+        # zero-value bounce-back lists folded into the fused step's pulls when that is
+        # exact (False: every list runs as a link-list launch; the results are the same)
+        self.fuse_bounce_back = True
         self.init_type = dico.get("inittype", "moments")
         self.init_data = dico.get("init", None)
         self._need_init = True
@@ -223,15 +226,69 @@ class Simulation:
                 # before the link lists: pylbm's wrap happens before its boundary methods
                 self._dev_links.insert(0, DeviceLinks(self.rt, "ghost_copy", self.container.F.layout,
                                                       self.wshift, flat=(st, ld)))
-        self._link_sets = (rtm.LbmLinkSet * max(1, len(self._dev_links)))(
-            *[d.link_set() for d in self._dev_links])
+        for d in self._dev_links:
+            d.fused = False
+        self.params.bbmask = None
+        if self.fuse_bounce_back:
+            self._fuse_bounce_back()
+        active = [d for d in self._dev_links if not d.fused]
+        self._active_links = active
+        self._link_sets = (rtm.LbmLinkSet * max(1, len(active)))(*[d.link_set() for d in active])
         # independent consecutive lists run in one launch (bit-identical results)
-        flags = merge_flags([(d.h_store, d.h_load, d.conflict) for d in self._dev_links])
+        flags = merge_flags([(d.h_store, d.h_load, d.conflict) for d in active])
         for ls, flag in zip(self._link_sets, flags):
             ls.merge_prev = flag if self.merge_links else 0
         for d in self._dev_links:
             d.h_store = d.h_load = None
         self._need_init = False
+
+    def _fuse_bounce_back(self):
+        """Zero-value bounce-back lists become a direction mask of the fused step.
+
+        A link f[k, x - v_k] = f[ksym, x] (pylbm/boundary.py:411-427, no value)
+        stores the slot that only the pull of population k at the fluid cell x
+        reads, so the step may read f[ksym, x] in its place.  That gives the same
+        interior result as long as no later list of the step reads or writes the
+        list's stored slots or stores its loaded slots (earlier lists run before it
+        either way).  Such lists leave the step's link sets; their slots in the
+        halo and in solid cells then keep their previous values (pylbm's read-back
+        of the interior never sees them)."""
+        devs = self._dev_links
+        sten = self.scheme.stencil
+        ksym_all = np.asarray(sten.get_symmetric())
+        v = np.asarray(sten.get_all_velocities())
+        halo = np.asarray(self.domain.stencil.vmax)
+        shape_in = tuple(int(n) for n in self.domain.shape_in)
+        mask = None
+        owner = None
+        for j, d in enumerate(devs):
+            m = d.method
+            if (isinstance(m, str) or m.kind.name != "bounce_back" or m._tbc or d.conflict or d.n == 0
+                    or d.d_rhs is not None or d.d_sign is not None or d.sign != 1.0):
+                continue
+            rows_s = np.asarray(m._store_rows())
+            rows_l = np.asarray(m.iload[0].T if m._fixed else m.iload[0])
+            k = rows_s[0]
+            x = rows_l[1:]
+            if not (np.array_equal(rows_l[0], ksym_all[k]) and np.array_equal(x, rows_s[1:] + v[k].T)):
+                continue
+            xi = x - halo[:, None]
+            if np.any(xi < 0) or np.any(xi >= np.asarray(shape_in)[:, None]):
+                continue
+            later = devs[j + 1:]
+            if any(np.isin(o.h_store, d.h_store).any() or np.isin(o.h_load, d.h_store).any()
+                   or np.isin(o.h_store, d.h_load).any() for o in later):
+                continue
+            if mask is None:
+                mask = np.zeros(int(np.prod(shape_in)), dtype=np.uint32)
+                owner = d
+            flat = np.ravel_multi_index(tuple(xi), shape_in)
+            np.bitwise_or.at(mask, flat, (np.uint32(1) << k.astype(np.uint32)))
+            d.fused = True
+        if mask is not None:
+            ptr = self.rt.upload(mask)
+            owner._ptrs.append(ptr)   # freed with the list's other device arrays
+            self.params.bbmask = ptr
 
     def initialization(self):
         coords = np.meshgrid(*self.domain.coords_halo, sparse=True, indexing="ij")
@@ -318,7 +375,8 @@ class Simulation:
             devs = devs[1:]
         for method, dev in zip(self.bc.methods, devs):
             self._update_feq(method, dev)
-            dev.apply(f.ptr)
+            if not dev.fused:
+                dev.apply(f.ptr)
 
     def _fused(self):
         F, Fn = self.container.F, self.container.Fnew
@@ -367,7 +425,7 @@ class Simulation:
         self._check(self.module.lbm_gen_run(ctypes.c_void_p(c.F.ptr), ctypes.c_void_p(c.Fnew.ptr),
                                             ctypes.byref(c.F.layout), ctypes.byref(self.vel),
                                             ctypes.byref(self.params), self._link_sets,
-                                            len(self._dev_links), int(nsteps), ghost, 1 if fold else 0,
+                                            len(self._active_links), int(nsteps), ghost, 1 if fold else 0,
                                             ctypes.byref(parity), self.rt.stream), "run")
         if parity.value:
             c.F, c.Fnew = c.Fnew, c.F

@@ -225,13 +225,41 @@ def test_merged_link_launches_bit_identical(hip, dtype, make):
     """Independent consecutive link lists applied in one launch (run()'s default) and
     one launch per list give the same run, bit for bit."""
     d = make()
-    a = Simulation(d, dtype=dtype)
+    a = Simulation(d, dtype=dtype, initialize=False)
     b = Simulation(d, dtype=dtype, initialize=False)
+    for sim in (a, b):
+        sim.fuse_bounce_back = False   # every list stays a link list: merging alone is tested
     b.merge_links = False
+    a._initialize()
     b._initialize()
-    n = len(a._dev_links)
+    n = len(a._active_links)
     assert any(a._link_sets[j].merge_prev for j in range(n))
     assert not any(b._link_sets[j].merge_prev for j in range(n))
     a.run(40)
     b.run(40)
     assert np.array_equal(a.container.F.fields_raw(), b.container.F.fields_raw())
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32", "float16"])
+@pytest.mark.parametrize("name", ["spheres", "karman", "d1q3_walls"])
+def test_fused_bounce_back_bit_identical(hip, dtype, name):
+    """Zero-value bounce-back lists folded into the step's pulls (the default) and run
+    as link-list launches give the same interior populations, bit for bit, through
+    one_time_step and through run()."""
+    d = {"spheres": lambda: cases.sphere_packing_d3q19(24, 16, 12, seed=5, solid_fraction=0.2),
+         "karman": lambda: cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0)),
+         "d1q3_walls": lambda: cases.d1q3_advection(16, walls=True)}[name]()
+    a = Simulation(d, dtype=dtype)
+    b = Simulation(d, dtype=dtype, initialize=False)
+    b.fuse_bounce_back = False
+    b._initialize()
+    assert not any(x.fused for x in b._dev_links)
+    if name == "spheres":
+        assert any(x.fused for x in a._dev_links)
+    for _ in range(3):
+        a.one_time_step()
+        b.one_time_step()
+    a.run(30)
+    b.run(30)
+    for k in range(a.ns):
+        assert np.array_equal(a.F[k], b.F[k]), k
