@@ -196,9 +196,10 @@ def test_make_input_target_and_apply_step(rt, elem):
                                                (32, 3, "bfloat16"), (64, 1, "bfloat16")])
 def test_fused_run_matches_layered_steps(rt, hidden, steps, elem):
     """lbm_sur_run (all layers of a step in one launch, halo recomputed in LDS) against
-    the layer-by-layer path: each path is deterministic, and they agree to the 16-bit
-    rounding of the hidden activations (a 1-ulp flip of one FP16 hidden value moves
-    the state by ~3e-4 of its increment, 4 u); the next inputs agree to one ulp."""
+    the layer-by-layer path: both accumulate every output in the same K order and
+    round every stored value through the same helpers, so the states and the next
+    inputs agree bit for bit; each path is deterministic, and the next input is the
+    normalised, unshuffled 16-bit value of the state to one rounding."""
     from pylbm_amd.surrogate import _f3
     Hf, Wf = 64, 256
     u = UNIT[elem]
@@ -223,13 +224,11 @@ def test_fused_run_matches_layered_steps(rt, hidden, steps, elem):
                         _dec(_download(rt, net.xin, (net.H, net.W, 64), _store(elem)), elem)))
     (sl, xl), (sf, xf) = results[0], results[1]
     assert np.array_equal(sl, results[2][0]) and np.array_equal(sf, results[3][0])
-    inc = np.abs(sl.astype(np.float64) - s0)
-    assert inc.max() > 0
-    diff = np.abs(sl.astype(np.float64) - sf)
-    assert diff.max() <= 4 * u * inc.max(), (diff.max(), inc.max())
-    assert np.mean(diff > 0) < 0.5
-    # each path's next input is the normalised, unshuffled 16-bit value of its own
-    # state, to one rounding (the FP32 product is formed in registers, maybe contracted)
+    assert np.abs(sl.astype(np.float64) - s0).max() > 0
+    assert np.array_equal(sl, sf)
+    assert np.array_equal(xl, xf)
+    # the next input is the normalised, unshuffled 16-bit value of the state, to one
+    # rounding (the FP32 value is rounded once more to E)
     from oracle import surrogate as osur
     sh = np.array([1.0, 0.0, 0.0, 0.0], np.float64)
     isc = 1.0 / np.array([2e-3, 0.05, 0.05, 1.0], np.float64)
