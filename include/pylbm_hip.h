@@ -170,13 +170,22 @@ typedef struct lbm_params {
   double cref[LBM_MAX_Q];   /* reference conserved moments (rest state) of the deviation formats */
   double dt;
   double t;
-  /* NULL, or one word per interior cell (outer, mid, inner order): bit k set means
-   * the pull of population k at this cell reads the cell's own population ksym
-   * instead of f[k, x - v_k].  That equals a zero-value bounce-back link
-   * f[k, x - v_k] = f[ksym, x] (pylbm/boundary.py:411-427) applied before the
-   * step, when no later link list of the step reads or writes its slots (the host
-   * checks this and then leaves the list out of the step's link sets). */
+  /* Link lists folded into the step's pulls.  bbmask: NULL, or one word per
+   * interior cell (outer, mid, inner order); bit k set means the pull of
+   * population k at this cell does not read f[k, x - v_k] but computes what the
+   * link f[k, x - v_k] = sign * f[ksym, x] + value would have stored there
+   * (bounce-back / anti-bounce-back, pylbm/boundary.py:411-427, 637-643), with the
+   * link kernel's arithmetic.  sign = 1 without a value unless the cell's
+   * bbtab_bits word (NULL: all zero) has bit k set; then entry
+   * bbtab_base[cell] + (number of lower set bits) of bbtab_sign / bbtab_rhs (the
+   * compute type: double for FP64 storage, float otherwise) holds sign and value.
+   * Exact when no later link list of the step reads or writes the folded links'
+   * slots: the host checks this and then leaves those lists out of the step. */
   const uint32_t* bbmask;
+  const uint32_t* bbtab_bits;
+  const uint32_t* bbtab_base;
+  const void* bbtab_sign;
+  const void* bbtab_rhs;
 } lbm_params;
 
 /* information on the generated module (ns, nconsm, dim) */

@@ -141,8 +141,9 @@ __global__ void __launch_bounds__(256) links_kernel(S* __restrict__ f,
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   // deviation formats: rhs already carries the rest-population shift, so the
-  // stored deviations are combined directly (w = 0)
-  C v = Codec<S>::load(f, load[t], (C)0);
+  // stored deviations are combined directly; load_dev / store with w = 0 keep
+  // every bit (a sign-1 link without rhs copies the stored value, -0 included)
+  C v = Codec<S>::load_dev(f, load[t], (C)0);
   const C a = sgn ? (C)sgn[t] : (C)sign;
   C r = a * v;
   if (rhs) r += rhs[t];
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(256) links_gather_kernel(const S* __restrict__
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const C a = sgn ? (C)sgn[t] : (C)sign;
-  C r = a * Codec<S>::load(f, load[t], (C)0);
+  C r = a * Codec<S>::load_dev(f, load[t], (C)0);
   if (rhs) r += rhs[t];
   work[t] = r;
 }
@@ -221,7 +222,7 @@ __global__ void __launch_bounds__(256) links_group_kernel(S* __restrict__ f,
   int j = 0;
   while (t >= G.end[j]) ++j;
   const int64_t i = t - (j ? G.end[j - 1] : 0);
-  C v = Codec<S>::load(f, G.load[j][i], (C)0);
+  C v = Codec<S>::load_dev(f, G.load[j][i], (C)0);
   const C a = G.sgn[j] ? (C)G.sgn[j][i] : (C)G.sign[j];
   C r = a * v;
   if (G.rhs[j]) r += G.rhs[j][i];

@@ -377,7 +377,11 @@ struct StepArgs {
   int32_t nin[3];        // interior extents
   int32_t halo[3];
   int32_t n1, rs, lead, _pad;      // rs: elements between consecutive rows
-  const uint32_t* bbmask;          // NULL or fused bounce-back directions per interior cell
+  const uint32_t* bbmask;          // NULL or folded link directions per interior cell
+  const uint32_t* bbtab_bits;      // NULL or the folded links with a sign / value table entry
+  const uint32_t* bbtab_base;
+  const C* bbtab_sign;
+  const C* bbtab_rhs;
   C omega[NS], one_minus[NS], wshift[NS];
   C cref[NCREF];
   C half_dt, t;
@@ -520,6 +524,10 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   a->half_dt = p ? (C)(0.5 * p->dt) : (C)0;
   a->t = p ? (C)p->t : (C)0;
   a->bbmask = p ? p->bbmask : nullptr;
+  a->bbtab_bits = p ? p->bbtab_bits : nullptr;
+  a->bbtab_base = p ? p->bbtab_base : nullptr;
+  a->bbtab_sign = p ? (const C*)p->bbtab_sign : nullptr;
+  a->bbtab_rhs = p ? (const C*)p->bbtab_rhs : nullptr;
   LBM_REQUIRE(a->bbmask == nullptr || BBSUB_OK, "this stencil has no symmetric pairs: no fused bounce-back");
   return 0;
 }
@@ -691,21 +699,36 @@ def generate_source(spec: KernelSpec) -> str:
     mvars = ["m%d" % i for i in range(ns)]
 
     # ---------------------------------------------------------- fused step
-    # fused zero-value bounce-back: where bit k of the cell's mask word is set, the
-    # pulled value of population k is the cell's own population ksym (the value
-    # the link f[k, x - v_k] = f[ksym, x] would have stored)
+    # folded link lists: where bit k of the cell's mask word is set, the pulled
+    # value of population k is what the link f[k, x - v_k] = sign f[ksym, x] + value
+    # would have stored, computed as links_kernel computes it: a copy of the stored
+    # value for sign 1 without a value, else decode_dev, r = sign v, r += value,
+    # encode (w = 0) with the sign and value of the cell's table entry
     bbsub = []
     if ksym is not None:
         bbsub = ["if constexpr (!FOLD) {",
                  "  if (a.bbmask) {",
-                 "    const uint32_t* bm = a.bbmask + ((int64_t)oo * a.nin[1] + mm) * a.nin[2] + i;",
+                 "    const int64_t mc = ((int64_t)oo * a.nin[1] + mm) * a.nin[2] + i;",
                  "#pragma unroll",
                  "    for (int j = 0; j < VEC; ++j) {",
-                 "      const uint32_t bits = bm[j];",
-                 "      if (bits) {"]
+                 "      const uint32_t bits = a.bbmask[mc + j];",
+                 "      if (bits) {",
+                 "        const uint32_t tb = a.bbtab_bits ? a.bbtab_bits[mc + j] : 0u;",
+                 "        const uint32_t tbase = tb ? a.bbtab_base[mc + j] : 0u;",
+                 "        (void)tbase;"]
         for k in range(ns):
             if ksym[k] != k:
-                bbsub.append("        if (bits & %du) L%d[j] = f[a.dst_off[%d] + cell + j];" % (1 << k, k, ksym[k]))
+                bit = 1 << k
+                bbsub += ["        if (bits & %du) {" % bit,
+                          "          L%d[j] = f[a.dst_off[%d] + cell + j];" % (k, ksym[k]),
+                          "          if (tb & %du) {" % bit,
+                          "            using CC = typename Codec<S>::C;",
+                          "            const uint32_t e_ = tbase + (uint32_t)__builtin_popcount(tb & %du);" % (bit - 1),
+                          "            CC r_ = a.bbtab_sign[e_] * Codec<S>::decode_dev(L%d[j], (CC)0);" % k,
+                          "            r_ += a.bbtab_rhs[e_];",
+                          "            L%d[j] = Codec<S>::encode(r_, (CC)0);" % k,
+                          "          }",
+                          "        }"]
         bbsub += ["      }", "    }", "  }", "}"]
     loads = ["S %s;" % ", ".join("L%d[VEC]" % k for k in range(ns))]
     loads.append("if (!FOLD) {")

@@ -26,7 +26,7 @@ import numpy as np
 import sympy as sp
 
 from . import runtime as rtm
-from .boundary import Boundary, DeviceLinks, merge_flags, unwritten_ghost_reads
+from .boundary import Boundary, DeviceLinks, host_lists, merge_flags, unwritten_ghost_reads
 from .domain import Domain
 from .generator.build import load_module
 from .monitoring import Monitor
@@ -154,9 +154,9 @@ class Simulation:
         # consecutive independent link lists share one launch in run() (False: one
         # launch per list; the results are the same)
         self.merge_links = True
-        # zero-value bounce-back lists folded into the fused step's pulls when that is
-        # exact (False: every list runs as a link-list launch; the results are the same)
-        self.fuse_bounce_back = True
+        # bounce-back / anti-bounce-back lists folded into the fused step's pulls when
+        # that is exact (False: every list runs as a link-list launch; same results)
+        self.fold_links = True
         self.init_type = dico.get("inittype", "moments")
         self.init_data = dico.get("init", None)
         self._need_init = True
@@ -228,9 +228,10 @@ class Simulation:
                                                       self.wshift, flat=(st, ld)))
         for d in self._dev_links:
             d.fused = False
-        self.params.bbmask = None
-        if self.fuse_bounce_back:
-            self._fuse_bounce_back()
+        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_sign", "bbtab_rhs"):
+            setattr(self.params, name, None)
+        if self.fold_links:
+            self._fold_links()
         active = [d for d in self._dev_links if not d.fused]
         self._active_links = active
         self._link_sets = (rtm.LbmLinkSet * max(1, len(active)))(*[d.link_set() for d in active])
@@ -242,32 +243,40 @@ class Simulation:
             d.h_store = d.h_load = None
         self._need_init = False
 
-    def _fuse_bounce_back(self):
-        """Zero-value bounce-back lists become a direction mask of the fused step.
+    def _fold_links(self):
+        """Bounce-back and anti-bounce-back lists folded into the fused step's pulls.
 
-        A link f[k, x - v_k] = f[ksym, x] (pylbm/boundary.py:411-427, no value)
-        stores the slot that only the pull of population k at the fluid cell x
-        reads, so the step may read f[ksym, x] in its place.  That gives the same
-        interior result as long as no later list of the step reads or writes the
-        list's stored slots or stores its loaded slots (earlier lists run before it
-        either way).  Such lists leave the step's link sets; their slots in the
-        halo and in solid cells then keep their previous values (pylbm's read-back
-        of the interior never sees them)."""
+        A link f[k, x - v_k] = sign f[ksym, x] + value (pylbm/boundary.py:411-427,
+        637-643) stores a slot that, within the interior update, only the pull of
+        population k at the fluid cell x reads, and it loads a fluid slot that no
+        list stores.  So the step may compute that value itself where it would read
+        the slot.  The host folds a list when every link has that form with x
+        interior, no other folded link has the same (k, x), and no later list of the
+        step reads or writes its stored slots (earlier lists run before the pull
+        either way).  Folded lists leave the step's link sets; the slots they would
+        have written, in the halo and in solid cells, keep their previous values
+        (the interior update never reads them)."""
         devs = self._dev_links
+        layout = self.container.F.layout
         sten = self.scheme.stencil
         ksym_all = np.asarray(sten.get_symmetric())
         v = np.asarray(sten.get_all_velocities())
         halo = np.asarray(self.domain.stencil.vmax)
         shape_in = tuple(int(n) for n in self.domain.shape_in)
-        mask = None
+        ncell = int(np.prod(shape_in))
+        mask = np.zeros(ncell, dtype=np.uint32)
+        tab = []   # (cell, k, sign, rhs) of the folded links that need a table entry
         owner = None
         for j, d in enumerate(devs):
             m = d.method
-            if (isinstance(m, str) or m.kind.name != "bounce_back" or m._tbc or d.conflict or d.n == 0
-                    or d.d_rhs is not None or d.d_sign is not None or d.sign != 1.0):
+            if (isinstance(m, str) or m.kind.name not in ("bounce_back", "anti_bounce_back") or m._tbc
+                    or d.conflict or d.n == 0):
                 continue
+            st, ld, sign, rhs, conflict, order = host_lists(m, layout, self.wshift)
             rows_s = np.asarray(m._store_rows())
             rows_l = np.asarray(m.iload[0].T if m._fixed else m.iload[0])
+            if order is not None:
+                rows_s, rows_l = rows_s[:, order], rows_l[:, order]
             k = rows_s[0]
             x = rows_l[1:]
             if not (np.array_equal(rows_l[0], ksym_all[k]) and np.array_equal(x, rows_s[1:] + v[k].T)):
@@ -275,20 +284,44 @@ class Simulation:
             xi = x - halo[:, None]
             if np.any(xi < 0) or np.any(xi >= np.asarray(shape_in)[:, None]):
                 continue
-            later = devs[j + 1:]
-            if any(np.isin(o.h_store, d.h_store).any() or np.isin(o.h_load, d.h_store).any()
-                   or np.isin(o.h_store, d.h_load).any() for o in later):
+            cell = np.ravel_multi_index(tuple(xi), shape_in)
+            bit = np.uint32(1) << k.astype(np.uint32)
+            if np.any(mask[cell] & bit):
                 continue
-            if mask is None:
-                mask = np.zeros(int(np.prod(shape_in)), dtype=np.uint32)
-                owner = d
-            flat = np.ravel_multi_index(tuple(xi), shape_in)
-            np.bitwise_or.at(mask, flat, (np.uint32(1) << k.astype(np.uint32)))
+            if any(np.isin(o.h_store, d.h_store).any() or np.isin(o.h_load, d.h_store).any()
+                   or np.isin(o.h_store, d.h_load).any() for o in devs[j + 1:]):
+                continue
+            np.bitwise_or.at(mask, cell, bit)
+            if d.d_rhs is not None or d.d_sign is not None or d.sign != 1.0:
+                # the link kernel adds rhs only when the list has one: -0.0 adds nothing
+                r = rhs if d.d_rhs is not None else np.full(d.n, -0.0)
+                tab.append((cell, k, sign, r))
             d.fused = True
-        if mask is not None:
-            ptr = self.rt.upload(mask)
-            owner._ptrs.append(ptr)   # freed with the list's other device arrays
-            self.params.bbmask = ptr
+            owner = owner or d
+        if owner is None:
+            return
+        ctype = np.float64 if self.storage == LBM_F64 else np.float32
+        up = []
+        self.params.bbmask = self.rt.upload(mask)
+        up.append(self.params.bbmask)
+        if tab:
+            cell = np.concatenate([t[0] for t in tab])
+            kk = np.concatenate([t[1] for t in tab])
+            sg = np.concatenate([t[2] for t in tab])
+            rh = np.concatenate([t[3] for t in tab])
+            o = np.lexsort((kk, cell))
+            cell, kk, sg, rh = cell[o], kk[o], sg[o], rh[o]
+            bits = np.zeros(ncell, dtype=np.uint32)
+            np.bitwise_or.at(bits, cell, np.uint32(1) << kk.astype(np.uint32))
+            base = np.zeros(ncell, dtype=np.uint32)
+            first_cell, first = np.unique(cell, return_index=True)
+            base[first_cell] = first.astype(np.uint32)
+            for name, arr in (("bbtab_bits", bits), ("bbtab_base", base), ("bbtab_sign", sg.astype(ctype)),
+                              ("bbtab_rhs", rh.astype(ctype))):
+                ptr = self.rt.upload(np.ascontiguousarray(arr))
+                setattr(self.params, name, ptr)
+                up.append(ptr)
+        owner._ptrs.extend(up)   # freed with the list's other device arrays
 
     def initialization(self):
         coords = np.meshgrid(*self.domain.coords_halo, sparse=True, indexing="ij")

@@ -228,7 +228,7 @@ def test_merged_link_launches_bit_identical(hip, dtype, make):
     a = Simulation(d, dtype=dtype, initialize=False)
     b = Simulation(d, dtype=dtype, initialize=False)
     for sim in (a, b):
-        sim.fuse_bounce_back = False   # every list stays a link list: merging alone is tested
+        sim.fold_links = False   # every list stays a link list: merging alone is tested
     b.merge_links = False
     a._initialize()
     b._initialize()
@@ -241,21 +241,25 @@ def test_merged_link_launches_bit_identical(hip, dtype, make):
 
 
 @pytest.mark.parametrize("dtype", ["float64", "float32", "float16"])
-@pytest.mark.parametrize("name", ["spheres", "karman", "d1q3_walls"])
-def test_fused_bounce_back_bit_identical(hip, dtype, name):
-    """Zero-value bounce-back lists folded into the step's pulls (the default) and run
-    as link-list launches give the same interior populations, bit for bit, through
-    one_time_step and through run()."""
+@pytest.mark.parametrize("name", ["spheres", "karman", "d1q3_walls", "lid"])
+def test_folded_links_bit_identical(hip, dtype, name):
+    """Bounce-back / anti-bounce-back lists folded into the step's pulls (the default:
+    the spheres' zero-value list, the lid cavity's list with the lid's values) and
+    run as link-list launches give the same interior populations, bit for bit,
+    through one_time_step and through run()."""
     d = {"spheres": lambda: cases.sphere_packing_d3q19(24, 16, 12, seed=5, solid_fraction=0.2),
          "karman": lambda: cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0)),
-         "d1q3_walls": lambda: cases.d1q3_advection(16, walls=True)}[name]()
+         "d1q3_walls": lambda: cases.d1q3_advection(16, walls=True),
+         "lid": lambda: cases.lid_cavity_d2q9(24)}[name]()
     a = Simulation(d, dtype=dtype)
     b = Simulation(d, dtype=dtype, initialize=False)
-    b.fuse_bounce_back = False
+    b.fold_links = False
     b._initialize()
     assert not any(x.fused for x in b._dev_links)
-    if name == "spheres":
+    if name in ("spheres", "lid"):
         assert any(x.fused for x in a._dev_links)
+    if name == "lid":
+        assert a.params.bbtab_rhs   # the lid's values go through the table
     for _ in range(3):
         a.one_time_step()
         b.one_time_step()
