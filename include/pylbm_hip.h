@@ -172,18 +172,20 @@ typedef struct lbm_params {
   double t;
   /* Link lists folded into the step's pulls.  bbmask: NULL, or one word per
    * interior cell (outer, mid, inner order); bit k set means the pull of
-   * population k at this cell does not read f[k, x - v_k] but computes what the
-   * link f[k, x - v_k] = sign * f[ksym, x] + value would have stored there
-   * (bounce-back / anti-bounce-back, pylbm/boundary.py:411-427, 637-643), with the
-   * link kernel's arithmetic.  sign = 1 without a value unless the cell's
-   * bbtab_bits word (NULL: all zero) has bit k set; then entry
-   * bbtab_base[cell] + (number of lower set bits) of bbtab_sign / bbtab_rhs (the
-   * compute type: double for FP64 storage, float otherwise) holds sign and value.
-   * Exact when no later link list of the step reads or writes the folded links'
-   * slots: the host checks this and then leaves those lists out of the step. */
+   * population k at this cell does not read f[k, x - v_k] but computes the value
+   * the step's link lists (pylbm/simulation.py:385-390; bounce-back,
+   * anti-bounce-back, Neumann, pylbm/boundary.py:411-427, 637-643, 777-890) would
+   * have left there, with the link kernel's arithmetic: a copy of f[ksym, x]
+   * unless bit 31 of the mask word is set (ns <= 31) and the cell's bbtab_bits
+   * word has bit k set; then entry
+   * e = bbtab_base[cell] + (number of lower set bits) gives
+   * sign[e] * f[bbtab_src[e]] + rhs[e] (sign / rhs in the compute type: double for
+   * FP64 storage, float otherwise; src a flat element index).  The host folds
+   * only what this reproduces exactly and leaves folded lists out of the step. */
   const uint32_t* bbmask;
   const uint32_t* bbtab_bits;
   const uint32_t* bbtab_base;
+  const uint32_t* bbtab_src;
   const void* bbtab_sign;
   const void* bbtab_rhs;
 } lbm_params;

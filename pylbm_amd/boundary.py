@@ -141,6 +141,21 @@ def _flat(rows, layout):
     return idx
 
 
+def _unflat(idx, layout):
+    """(k, o, m, i) of flat element indices: population and canonical halo-inclusive
+    coordinates (outer, mid, inner), the inverse of _flat."""
+    idx = np.asarray(idx, dtype=np.int64)
+    if layout.pop_stride >= layout.row_stride:      # population planes (one row: equal)
+        k = idx // layout.pop_stride
+        r = idx % layout.pop_stride - layout.lead
+        row, i = r // layout.row_stride, r % layout.row_stride
+    else:                                           # rows of all populations interleaved
+        r = idx - layout.lead
+        row, rem = r // layout.row_stride, r % layout.row_stride
+        k, i = rem // layout.pop_stride, rem % layout.pop_stride
+    return k, row // layout.n[1], row % layout.n[1], i
+
+
 def _population(idx, layout):
     """Population index of flat element indices (either arrangement of the layout)."""
     if layout.pop_stride > layout.row_stride:
@@ -173,6 +188,47 @@ def _resolve_sequential(store, load, sign, rhs):
 
 
 MAX_LINK_GROUP = 8  # lists per launch of lbm_links_apply_sets
+
+
+def resolve_step(lists):
+    """The step's link lists, in order, as one value per stored slot.
+
+    lists: [(store, load, sign, rhs or None)] (flat indices; rhs None: the list
+    adds nothing).  Each list reads the values the lists before it left (its loads
+    are looked up among their stores) and the last store of a slot wins.  Returns
+    (slot, origin, sign, rhs, has_rhs, two_rhs), sorted by slot: the final value of
+    slot is sign * f[origin] + rhs (rhs only if has_rhs) with origin a slot no list
+    stores; two_rhs marks chains that applied two rhs (their value needs two
+    roundings, which sign * f + rhs does not reproduce)."""
+    slot = np.zeros(0, np.int64)
+    org = np.zeros(0, np.int64)
+    sg = np.zeros(0)
+    rh = np.zeros(0)
+    has = np.zeros(0, bool)
+    bad = np.zeros(0, bool)
+    for st, ld, s1, r1 in lists:
+        st = np.asarray(st, np.int64)
+        ld = np.asarray(ld, np.int64)
+        s1 = np.asarray(s1, np.float64)
+        h1 = np.full(st.size, r1 is not None)
+        r1 = np.asarray(r1, np.float64) if r1 is not None else np.zeros(st.size)
+        o2, s2, r2, h2, b2 = ld.copy(), s1.copy(), r1.copy(), h1.copy(), np.zeros(st.size, bool)
+        if slot.size:
+            pos = np.minimum(np.searchsorted(slot, ld), slot.size - 1)
+            hit = slot[pos] == ld
+            p = pos[hit]
+            o2[hit] = org[p]
+            s2[hit] = s1[hit] * sg[p]
+            # s1 (sg f + rh) + r1, exact when at most one of rh, r1 applies
+            r2[hit] = np.where(has[p], s1[hit] * rh[p], 0.0) + np.where(h1[hit], r1[hit], 0.0)
+            h2[hit] = h1[hit] | has[p]
+            b2[hit] = bad[p] | (h1[hit] & has[p])
+        # the stores of this list replace earlier stores of the same slots
+        cat = [np.concatenate(a) for a in ((slot, st), (org, o2), (sg, s2), (rh, r2), (has, h2), (bad, b2))]
+        uniq, first = np.unique(cat[0][::-1], return_index=True)
+        keep = cat[0].size - 1 - first
+        slot, org, sg, rh, has, bad = uniq, cat[1][keep], cat[2][keep], cat[3][keep], cat[4][keep], cat[5][keep]
+    return slot, org, sg, rh, has, bad
 
 
 def merge_flags(lists):
@@ -252,8 +308,10 @@ class DeviceLinks:
         st, ld, sign, rhs, self.conflict, self.order = host_lists(method, layout, wshift, flat)
         self.n = st.size
         self.rhs_host = rhs
-        # host copies of the final index lists, for merge_flags (dropped after use)
+        # host copies of the final lists, for merge_flags and link folding (dropped after use)
         self.h_store, self.h_load = st, ld
+        self.h_sign = sign
+        self.h_rhs = rhs if np.any(rhs != 0) else None
         ctype = np.float64 if layout.storage == LBM_F64 else np.float32
         self._ptrs = []
         self.d_store = self._up(st.astype(np.uint32))

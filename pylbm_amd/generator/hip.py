@@ -380,6 +380,7 @@ struct StepArgs {
   const uint32_t* bbmask;          // NULL or folded link directions per interior cell
   const uint32_t* bbtab_bits;      // NULL or the folded links with a sign / value table entry
   const uint32_t* bbtab_base;
+  const uint32_t* bbtab_src;
   const C* bbtab_sign;
   const C* bbtab_rhs;
   C omega[NS], one_minus[NS], wshift[NS];
@@ -526,6 +527,7 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   a->bbmask = p ? p->bbmask : nullptr;
   a->bbtab_bits = p ? p->bbtab_bits : nullptr;
   a->bbtab_base = p ? p->bbtab_base : nullptr;
+  a->bbtab_src = p ? p->bbtab_src : nullptr;
   a->bbtab_sign = p ? (const C*)p->bbtab_sign : nullptr;
   a->bbtab_rhs = p ? (const C*)p->bbtab_rhs : nullptr;
   LBM_REQUIRE(a->bbmask == nullptr || BBSUB_OK, "this stencil has no symmetric pairs: no fused bounce-back");
@@ -701,9 +703,10 @@ def generate_source(spec: KernelSpec) -> str:
     # ---------------------------------------------------------- fused step
     # folded link lists: where bit k of the cell's mask word is set, the pulled
     # value of population k is what the link f[k, x - v_k] = sign f[ksym, x] + value
-    # would have stored, computed as links_kernel computes it: a copy of the stored
-    # value for sign 1 without a value, else decode_dev, r = sign v, r += value,
-    # encode (w = 0) with the sign and value of the cell's table entry
+    # would have stored, computed as links_kernel computes it: a copy of f[ksym, x]
+    # for sign 1 without a value, else decode_dev of the table entry's source,
+    # r = sign v, r += value, encode (w = 0); bit 31 of the mask word marks the
+    # cells that have table entries (ns <= 31)
     bbsub = []
     if ksym is not None:
         bbsub = ["if constexpr (!FOLD) {",
@@ -713,7 +716,7 @@ def generate_source(spec: KernelSpec) -> str:
                  "    for (int j = 0; j < VEC; ++j) {",
                  "      const uint32_t bits = a.bbmask[mc + j];",
                  "      if (bits) {",
-                 "        const uint32_t tb = a.bbtab_bits ? a.bbtab_bits[mc + j] : 0u;",
+                 "        const uint32_t tb = (bits & 0x80000000u) ? a.bbtab_bits[mc + j] : 0u;",
                  "        const uint32_t tbase = tb ? a.bbtab_base[mc + j] : 0u;",
                  "        (void)tbase;"]
         for k in range(ns):
@@ -724,7 +727,7 @@ def generate_source(spec: KernelSpec) -> str:
                           "          if (tb & %du) {" % bit,
                           "            using CC = typename Codec<S>::C;",
                           "            const uint32_t e_ = tbase + (uint32_t)__builtin_popcount(tb & %du);" % (bit - 1),
-                          "            CC r_ = a.bbtab_sign[e_] * Codec<S>::decode_dev(L%d[j], (CC)0);" % k,
+                          "            CC r_ = a.bbtab_sign[e_] * Codec<S>::decode_dev(f[a.bbtab_src[e_]], (CC)0);",
                           "            r_ += a.bbtab_rhs[e_];",
                           "            L%d[j] = Codec<S>::encode(r_, (CC)0);" % k,
                           "          }",

@@ -50,8 +50,8 @@ class LbmParams(ctypes.Structure):
     _fields_ = [("omega", ctypes.c_double * MAX_Q), ("one_minus", ctypes.c_double * MAX_Q),
                 ("wshift", ctypes.c_double * MAX_Q), ("cref", ctypes.c_double * MAX_Q),
                 ("dt", ctypes.c_double), ("t", ctypes.c_double), ("bbmask", ctypes.c_void_p),
-                ("bbtab_bits", ctypes.c_void_p), ("bbtab_base", ctypes.c_void_p), ("bbtab_sign", ctypes.c_void_p),
-                ("bbtab_rhs", ctypes.c_void_p)]
+                ("bbtab_bits", ctypes.c_void_p), ("bbtab_base", ctypes.c_void_p), ("bbtab_src", ctypes.c_void_p),
+                ("bbtab_sign", ctypes.c_void_p), ("bbtab_rhs", ctypes.c_void_p)]
 
 
 class LbmLinkSet(ctypes.Structure):

@@ -26,7 +26,8 @@ import numpy as np
 import sympy as sp
 
 from . import runtime as rtm
-from .boundary import Boundary, DeviceLinks, host_lists, merge_flags, unwritten_ghost_reads
+from .boundary import Boundary, DeviceLinks, host_lists, merge_flags, resolve_step, unwritten_ghost_reads
+from .boundary import _unflat as _unflat_fn
 from .domain import Domain
 from .generator.build import load_module
 from .monitoring import Monitor
@@ -37,6 +38,11 @@ from .utils import itemproperty
 __all__ = ["Simulation"]
 
 LBM_F64 = rtm.LBM_F64
+# Largest table (links that are not a copy of the cell's own ksym population) the
+# whole-step fold may build: table entries are scattered reads inside the step's
+# waves, and 126,620 of them spread over C4's waves cost 37 us per step against a
+# 4.5 us launch of the same copies (MI355X); smaller tables save their launches.
+FOLD_TABLE_MAX = 1 << 16
 
 
 def _canon_velocities(spec, box_label=None):
@@ -228,7 +234,7 @@ class Simulation:
                                                       self.wshift, flat=(st, ld)))
         for d in self._dev_links:
             d.fused = False
-        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_sign", "bbtab_rhs"):
+        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_src", "bbtab_sign", "bbtab_rhs"):
             setattr(self.params, name, None)
         if self.fold_links:
             self._fold_links()
@@ -244,18 +250,66 @@ class Simulation:
         self._need_init = False
 
     def _fold_links(self):
-        """Bounce-back and anti-bounce-back lists folded into the fused step's pulls.
+        """Link lists folded into the fused step's pulls.
 
-        A link f[k, x - v_k] = sign f[ksym, x] + value (pylbm/boundary.py:411-427,
-        637-643) stores a slot that, within the interior update, only the pull of
-        population k at the fluid cell x reads, and it loads a fluid slot that no
-        list stores.  So the step may compute that value itself where it would read
-        the slot.  The host folds a list when every link has that form with x
-        interior, no other folded link has the same (k, x), and no later list of the
-        step reads or writes its stored slots (earlier lists run before the pull
-        either way).  Folded lists leave the step's link sets; the slots they would
-        have written, in the halo and in solid cells, keep their previous values
-        (the interior update never reads them)."""
+        Within the interior update a stored link slot (k, y) is read only by the
+        pull of population k at x = y + v_k, so the step may compute the value the
+        lists would have left there instead of reading it.  First the whole step
+        is tried (_fold_step); failing that, single bounce-back / anti-bounce-back
+        lists (_fold_lists).  Folded lists leave the step's link sets; the slots
+        they would have written, in the halo and in solid cells, keep their
+        previous values (the interior update never reads them)."""
+        if self.ns > 31:
+            return   # bit 31 of a mask word flags the cells with table entries
+        if not self._fold_step():
+            self._fold_lists()
+
+    def _fold_layout(self):
+        L = self.container.F.layout
+        n = np.array([L.n[a] for a in range(3)], dtype=np.int64)
+        h = np.array([L.halo[a] for a in range(3)], dtype=np.int64)
+        return L, n, h, n - 2 * h
+
+    def _fold_step(self):
+        """Every list of the step, resolved in order into one value per stored slot
+        (boundary.resolve_step): sign * f[origin] + rhs of a slot no list stores.
+        That reproduces the launches' result exactly when no chain applies two rhs
+        (a copy or a sign flip does not round); otherwise, with a time-dependent
+        list, or when more than FOLD_TABLE_MAX links would need a table entry,
+        nothing is folded here."""
+        devs = self._dev_links
+        if not devs or any(not isinstance(d.method, str) and d.method._tbc for d in devs):
+            return False
+        L, n, h, nin = self._fold_layout()
+        slot, org, sg, rh, has, bad = resolve_step([(d.h_store, d.h_load, d.h_sign, d.h_rhs) for d in devs])
+        k, o, m, i = _unflat_fn(slot, L)
+        vel = np.array([[self.vel.v[q][a] for a in range(3)] for q in range(self.ns)], dtype=np.int64)
+        x = np.stack([o, m, i]) + vel[k].T
+        inside = np.all((x >= h[:, None]) & (x < (n - h)[:, None]), axis=0)
+        if np.any(bad & inside):
+            return False
+        k, x, org, sg, rh, has = k[inside], x[:, inside], org[inside], sg[inside], rh[inside], has[inside]
+        xi = x - h[:, None]
+        cell = (xi[0] * nin[1] + xi[1]) * nin[2] + xi[2]
+        ksym = np.asarray(self.scheme.stencil.get_symmetric())[k]
+        own = ksym * L.pop_stride + (x[0] * L.n[1] + x[1]) * L.row_stride + L.lead + x[2]
+        copy = (org == own) & (sg == 1.0) & ~has
+        if np.count_nonzero(~copy) > FOLD_TABLE_MAX:
+            return False
+        mask = np.zeros(int(np.prod(nin)), dtype=np.uint32)
+        np.bitwise_or.at(mask, cell, np.uint32(1) << k.astype(np.uint32))
+        t = ~copy
+        self._upload_fold(mask, (cell[t], k[t], org[t], sg[t], np.where(has[t], rh[t], -0.0)), devs[0])
+        for d in devs:
+            d.fused = True
+        return True
+
+    def _fold_lists(self):
+        """Single bounce-back / anti-bounce-back lists: every link is
+        f[k, x - v_k] = sign f[ksym, x] + value with x interior, no other folded link
+        targets the same (k, x), and no later list of the step loads or stores the
+        list's stored slots or stores its loaded slots (earlier lists run before
+        the pull either way)."""
         devs = self._dev_links
         layout = self.container.F.layout
         sten = self.scheme.stencil
@@ -263,9 +317,8 @@ class Simulation:
         v = np.asarray(sten.get_all_velocities())
         halo = np.asarray(self.domain.stencil.vmax)
         shape_in = tuple(int(n) for n in self.domain.shape_in)
-        ncell = int(np.prod(shape_in))
-        mask = np.zeros(ncell, dtype=np.uint32)
-        tab = []   # (cell, k, sign, rhs) of the folded links that need a table entry
+        mask = np.zeros(int(np.prod(shape_in)), dtype=np.uint32)
+        tab = []
         owner = None
         for j, d in enumerate(devs):
             m = d.method
@@ -292,35 +345,37 @@ class Simulation:
                    or np.isin(o.h_store, d.h_load).any() for o in devs[j + 1:]):
                 continue
             np.bitwise_or.at(mask, cell, bit)
-            if d.d_rhs is not None or d.d_sign is not None or d.sign != 1.0:
+            if d.h_rhs is not None or d.d_sign is not None or d.sign != 1.0:
                 # the link kernel adds rhs only when the list has one: -0.0 adds nothing
-                r = rhs if d.d_rhs is not None else np.full(d.n, -0.0)
-                tab.append((cell, k, sign, r))
+                tab.append((cell, k, ld, sign, rhs if d.h_rhs is not None else np.full(d.n, -0.0)))
             d.fused = True
             owner = owner or d
-        if owner is None:
-            return
-        ctype = np.float64 if self.storage == LBM_F64 else np.float32
+        if owner is not None:
+            cat = tuple(np.concatenate([t[q] for t in tab]) for q in range(5)) if tab else None
+            self._upload_fold(mask, cat, owner)
+
+    def _upload_fold(self, mask, tab, owner):
+        """Device copies of the fold mask and table (cell, k, src, sign, rhs) -> params."""
         up = []
-        self.params.bbmask = self.rt.upload(mask)
-        up.append(self.params.bbmask)
-        if tab:
-            cell = np.concatenate([t[0] for t in tab])
-            kk = np.concatenate([t[1] for t in tab])
-            sg = np.concatenate([t[2] for t in tab])
-            rh = np.concatenate([t[3] for t in tab])
+        if tab is not None and tab[0].size:
+            cell, kk, src, sg, rh = tab
             o = np.lexsort((kk, cell))
-            cell, kk, sg, rh = cell[o], kk[o], sg[o], rh[o]
-            bits = np.zeros(ncell, dtype=np.uint32)
+            cell, kk, src, sg, rh = cell[o], kk[o], src[o], sg[o], rh[o]
+            bits = np.zeros(mask.size, dtype=np.uint32)
             np.bitwise_or.at(bits, cell, np.uint32(1) << kk.astype(np.uint32))
-            base = np.zeros(ncell, dtype=np.uint32)
+            base = np.zeros(mask.size, dtype=np.uint32)
             first_cell, first = np.unique(cell, return_index=True)
             base[first_cell] = first.astype(np.uint32)
-            for name, arr in (("bbtab_bits", bits), ("bbtab_base", base), ("bbtab_sign", sg.astype(ctype)),
-                              ("bbtab_rhs", rh.astype(ctype))):
+            mask = mask.copy()
+            mask[first_cell] |= np.uint32(1 << 31)   # these cells read their table word
+            ctype = np.float64 if self.storage == LBM_F64 else np.float32
+            for name, arr in (("bbtab_bits", bits), ("bbtab_base", base), ("bbtab_src", src.astype(np.uint32)),
+                              ("bbtab_sign", sg.astype(ctype)), ("bbtab_rhs", rh.astype(ctype))):
                 ptr = self.rt.upload(np.ascontiguousarray(arr))
                 setattr(self.params, name, ptr)
                 up.append(ptr)
+        self.params.bbmask = self.rt.upload(mask)
+        up.append(self.params.bbmask)
         owner._ptrs.extend(up)   # freed with the list's other device arrays
 
     def initialization(self):

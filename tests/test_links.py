@@ -103,6 +103,23 @@ def test_resolve_sequential_matches_loop():
         assert np.unique(s2).size == s2.size
 
 
+def test_unflat_inverts_flat():
+    """_unflat recovers (k, canonical coordinates) of _flat indices in both arrangements."""
+    from pylbm_amd.boundary import _flat, _unflat
+    from pylbm_amd.runtime import LBM_F32, LBM_F64
+    from pylbm_amd.storage import make_layout
+    rng = np.random.default_rng(3)
+    for shape, ns in (((7, 6, 9), 19), ((11, 13), 9), ((17,), 3)):
+        for inter in (False, True):
+            for st in (LBM_F32, LBM_F64):
+                L = make_layout(ns, shape, (1,) * len(shape), st, interleaved=inter)
+                rows = np.stack([rng.integers(0, ns, 200)] + [rng.integers(0, n, 200) for n in shape])
+                k, o, m, i = _unflat(_flat(rows, L), L)
+                canon = [np.zeros(200, np.int64)] * (3 - len(shape)) + [rows[1 + d] for d in range(len(shape))]
+                assert np.array_equal(k, rows[0])
+                assert np.array_equal(o, canon[0]) and np.array_equal(m, canon[1]) and np.array_equal(i, canon[2])
+
+
 def test_flat_indices_both_arrangements():
     """Flat element indices of link rows and their population, for planes and
     interleaved rows, against a direct numpy index computation."""
@@ -279,3 +296,45 @@ def test_merged_link_groups_equal_sequential(name):
     assert np.array_equal(seq, grp)
     if name in ("karman", "vectorial"):
         assert sum(flags) > 0, flags
+
+
+@pytest.mark.parametrize("name", ["spheres", "lid", "d1q3_walls", "karman", "vectorial", "neumann_periodic"])
+def test_resolve_step_equals_sequential_lists(name):
+    """The step's lists (ghost copy list first) applied one after another to a random
+    FP64 array leave, in every slot they store, exactly sign * f0[origin] + rhs
+    from resolve_step, for every chain that applied at most one rhs."""
+    from oracle.lbm import Simulation as O
+    from pylbm_amd.boundary import host_lists, resolve_step, unwritten_ghost_reads
+    d = {"spheres": lambda: cases.sphere_packing_d3q19(20, 14, 12, seed=3, solid_fraction=0.25),
+         "lid": lambda: cases.lid_cavity_d2q9(12),
+         "d1q3_walls": lambda: cases.d1q3_advection(10, walls=True),
+         "karman": lambda: cases.karman_d2q9(40, 20, radius=4.0, center=(10.0, 10.0)),
+         "vectorial": lambda: cases.d2q5q9_vectorial(12, walls=True),
+         "neumann_periodic": _neumann_periodic}[name]()
+    o = O(d)
+    dom, pm = _product(d)
+    for m in pm:
+        m.fix_iload()
+    lay = _HostLayout(o.ns, o.F.shape[1:])
+    lists = []
+    st, ld = unwritten_ghost_reads(dom, dom.stencil, pm, lay)
+    if st.size:
+        lists.append(host_lists("ghost_copy", lay, None, flat=(st, ld)))
+    rng = np.random.default_rng(2)
+    for m in pm:
+        m.rhs = rng.normal(size=m.rhs.shape)
+        lists.append(host_lists(m, lay, None))
+    F0 = rng.normal(size=o.F.shape)
+    seq = F0.copy()
+    for lst in lists:
+        _apply(seq, lst)
+    slot, org, sg, rh, has, bad = resolve_step([(x[0], x[1], x[2], x[3] if np.any(x[3] != 0) else None)
+                                                for x in lists])
+    flat0, flat1 = F0.reshape(-1), seq.reshape(-1)
+    val = sg * flat0[org] + np.where(has, rh, 0.0)
+    assert slot.size > 0
+    assert np.array_equal(val[~bad], flat1[slot][~bad])
+    # slots no list stores are untouched
+    untouched = np.ones(flat0.size, bool)
+    untouched[slot] = False
+    assert np.array_equal(flat0[untouched], flat1[untouched])
