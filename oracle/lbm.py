@@ -15,13 +15,20 @@ Follows:
                                                        pylbm/algorithm/base.py:439-495
   * read-back m = M f on the full domain, interior
     view                                               pylbm/simulation.py:215-224
+  * time-dependent boundary values ("time_bc"): the
+    value function is called with t = 0 at set-up and
+    with the current t before every step's links,
+    then equilibrium + m2f -> feq -> rhs               pylbm/boundary.py:287-321,
+                                                       pylbm/simulation.py:384-390
 
 Pinned by the known answers in tests/test_oracle_lbm.py: the rest state is a
 fixed point; with every non-conserved rate 0 a step is the exact permutation
 f_k(x, t+1) = f_k(x - v_k, t) on a periodic box; with every rate 1 the
 post-collision state is M^-1 EQ(m_c); mass and momentum are conserved to
 round-off on periodic boxes (math.fsum); bounce-back on resting walls
-conserves mass.  The C1-C5 trajectories themselves have no reference
+conserves mass; a time-dependent value whose function ignores t gives the
+static value's run bit for bit, and one that switches at t0 gives the static
+runs' states before t0.  The C1-C5 trajectories themselves have no reference
 fixture, so this driver's end-to-end output is "parity unpinned" beyond
 those checks.
 """
@@ -111,15 +118,21 @@ class Simulation:
             raise NotImplementedError("oracle: distributions init")
         self.Fnew[...] = self.F
 
-    def _prepare_rhs(self, meth):
+    def _prepare_rhs(self, meth, t=None):
+        """feq and rhs of one link method; t=None: set-up (every value, time-dependent
+        ones at t = 0), else only the time-dependent values, at time t."""
         bcd = self.dico["boundary_conditions"]
-        meth.feq = np.zeros((self.ns, meth.istore.shape[1]))
+        if t is None:
+            meth.feq = np.zeros((self.ns, meth.istore.shape[1]))
+            meth.timed = False
         for key in np.unique(meth.ilabel):
             value = bcd[key].get("value")
             if value is None:
                 continue
-            if bcd[key].get("time_bc", False):
-                raise NotImplementedError("oracle: time-dependent boundary values")
+            timed = bool(bcd[key].get("time_bc", False))
+            meth.timed = meth.timed or timed
+            if t is not None and not timed:
+                continue
             idx = np.where(meth.ilabel == key)
             n = idx[0].size
             k = meth.istore[0, idx]
@@ -136,7 +149,10 @@ class Simulation:
             m = np.zeros(shp)
             f = np.zeros(shp)
             func, args = (value[0], tuple(value[1])) if isinstance(value, tuple) else (value, ())
-            func(f, _Moments(m, self.scheme.consm), *(coords + args))
+            if timed:
+                func(f, _Moments(m, self.scheme.consm), 0.0 if t is None else t, *(coords + args))
+            else:
+                func(f, _Moments(m, self.scheme.consm), *(coords + args))
             m = self._eq_full(m)
             f = self._m2f(m)
             meth.feq[:, idx[0]] = f.reshape(self.ns, n)
@@ -198,6 +214,8 @@ class Simulation:
     def one_time_step(self):
         self.ghost_fill(self.F)
         for meth in self.methods:
+            if getattr(meth, "timed", False):
+                self._prepare_rhs(meth, t=self.t)
             apply_links(self.F, meth)
         self.pull_step(self.F, self.Fnew)
         self.F, self.Fnew = self.Fnew, self.F

@@ -267,3 +267,33 @@ def test_folded_links_bit_identical(hip, dtype, name):
     b.run(30)
     for k in range(a.ns):
         assert np.array_equal(a.F[k], b.F[k]), k
+
+
+def test_time_dependent_lid_fp64_parity(hip):
+    """A lid velocity ramped in time ("time_bc": the value function gets t, re-evaluated
+    before every step's links, pylbm/boundary.py:307-321): the list is not folded
+    and runs with its refreshed values; FP64 parity with the oracle."""
+    def ramp(t):
+        return 0.05 * min(1.0, t / 10.0)
+
+    def make():
+        d = cases.lid_cavity_d2q9(32)
+
+        def lid(f, m, t, x, y):
+            m[cases.RHO] = 1.0
+            m[cases.QX] = ramp(t)
+            m[cases.QY] = 0.0
+
+        d["boundary_conditions"][1] = {"method": {0: d["boundary_conditions"][1]["method"][0]}, "value": lid,
+                                       "time_bc": True}
+        return d
+
+    sim, ref = Simulation(make()), Oracle(make())
+    assert not any(x.fused for x in sim._dev_links)
+    for _ in range(5):
+        sim.one_time_step()
+        ref.one_time_step()
+    sim.run(35)
+    for _ in range(35):
+        ref.one_time_step()
+    _compare(sim, ref, [cases.RHO, cases.QX, cases.QY], TOL64)

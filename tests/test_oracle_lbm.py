@@ -122,3 +122,52 @@ def test_fsum_oracle():
     o = O(_tgv(6))
     m = o.moments()
     assert o.conserved_sums()[0] == math.fsum(m[0].ravel())
+
+
+def _timed_lid(n, u_of_t):
+    """Lid cavity whose lid value is time-dependent ("time_bc"): m[QX] = u_of_t(t)."""
+    d = cases.lid_cavity_d2q9(n)
+
+    def lid(f, m, t, x, y):
+        m[cases.RHO] = 1.0
+        m[cases.QX] = u_of_t(t)
+        m[cases.QY] = 0.0
+
+    d["boundary_conditions"][1] = {"method": {0: d["boundary_conditions"][1]["method"][0]}, "value": lid,
+                                   "time_bc": True}
+    return d
+
+
+def test_time_dependent_value_ignoring_t_equals_static():
+    """A time-dependent lid whose function ignores t runs exactly like the static lid."""
+    a = O(cases.lid_cavity_d2q9(12))
+    b = O(_timed_lid(12, lambda t: 0.05))
+    for _ in range(15):
+        a.one_time_step()
+        b.one_time_step()
+    assert np.array_equal(a.F, b.F)
+
+
+def test_time_dependent_value_switch():
+    """A lid that switches from 0.05 to 0.02 at t0 = 6 follows the static 0.05 run
+    up to t0 and then the static 0.02 run started from its state at t0: the value
+    is taken at the step's start time (pylbm/simulation.py:384-390)."""
+    t0 = 6
+    timed = O(_timed_lid(12, lambda t: 0.05 if t < t0 else 0.02))
+    fast = O(cases.lid_cavity_d2q9(12, u_lid=0.05))
+    for _ in range(t0):
+        timed.one_time_step()
+        fast.one_time_step()
+    assert np.array_equal(timed.F, fast.F)
+    # same viscosity (nu depends on u_lid): the 0.02 lid on the 0.05 case's scheme
+    d = cases.lid_cavity_d2q9(12, u_lid=0.05)
+    d["boundary_conditions"][1]["value"] = _timed_lid(12, lambda t: 0.02)["boundary_conditions"][1]["value"]
+    d["boundary_conditions"][1]["time_bc"] = True
+    slow = O(d)
+    slow.F[...] = timed.F
+    slow.t, slow.nt = timed.t, timed.nt
+    for _ in range(5):
+        timed.one_time_step()
+        slow.one_time_step()
+    assert np.array_equal(timed.F, slow.F)
+    assert not np.array_equal(timed.F, fast.F)
