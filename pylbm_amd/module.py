@@ -1,0 +1,99 @@
+"""The generator module object pylbm's ``call_genfunction`` would call for backend 'hip'.
+
+pylbm calls its generated routines by keyword: ``call_genfunction``
+(pylbm/symbolic.py:288-299) reads ``inspect.getfullargspec(func).args`` and takes
+each argument by name from ``BaseAlgorithm._get_args`` (pylbm/algorithm/base.py:
+630-667: ``f, fnew, m, t, dt, ...``) or ``BoundaryMethod._get_args``
+(pylbm/boundary.py:358-376: ``f, istore, iload0, rhs, ncond, ...``).
+:class:`HipModule` exposes the routines of the hot path under those names and
+argument names, over the C ABI of include/pylbm_hip.h:
+
+* ``one_time_step(f, fnew, t, dt)``       generated one_time_step (pull.py:11-59)
+* ``f2m(f, m)``, ``m2f(m, f)``, ``equilibrium(m)``   generated routines (base.py:298-402)
+* ``update(f)``                           Array.update: periodic wrap of every
+                                          halo slot (storage.py:306-367)
+* ``bounce_back``, ``anti_bounce_back``   (f, istore, iload0, rhs, ncond)
+                                          (boundary.py:430-469, 646-684)
+* ``neumann``, ``neumannx/y/z``           (f, istore, iload0, ncond) (boundary.py:787-895)
+
+Buffers are device pointers (integers); ``istore`` / ``iload0`` are flat uint32
+element indices of the population layout (``pylbm_amd.boundary._flat`` converts
+pylbm's ``(k, ix, iy[, iz])`` rows), ``rhs`` is in the compute type (double for
+FP64 storage, float otherwise) or None.  A list that reads slots it writes must
+be given resolved (``pylbm_amd.boundary.host_lists``) together with a workspace,
+the keyword-only ``work`` of ncond values (keyword-only, so that the argument
+names pylbm reads are exactly its own).  Every routine is asynchronous on the module's stream
+and raises RuntimeError with the library's message on failure.
+"""
+import ctypes
+
+from . import runtime as rtm
+
+__all__ = ["HipModule"]
+
+
+class HipModule:
+    """pylbm's routine names for one generated scheme module and one population layout."""
+
+    def __init__(self, gen, layout, mlayout, vel, params, rt=None):
+        self.rt = rt or rtm.get()
+        self.core = self.rt.lib
+        self.gen = gen
+        self.layout, self.mlayout, self.vel, self.params = layout, mlayout, vel, params
+        self.stream = self.rt.stream
+
+    @classmethod
+    def from_simulation(cls, sim):
+        """The module of a pylbm_amd Simulation: its generated module, layouts,
+        velocities and relaxation parameters (folded link lists are not used)."""
+        params = rtm.LbmParams.from_buffer_copy(sim.params)
+        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_src", "bbtab_sign", "bbtab_rhs"):
+            setattr(params, name, None)
+        return cls(sim.module, sim.container.F.layout, sim.container.m.layout, sim.vel, params, sim.rt)
+
+    def _ok(self, rc, what):
+        if rc:
+            raise RuntimeError("%s: %s" % (what, self.core.lbm_last_error().decode()))
+
+    @staticmethod
+    def _p(x):
+        return ctypes.c_void_p(int(x)) if x is not None else None
+
+    # ------------------------------------------------ algorithm (base.py:603-628)
+    def one_time_step(self, f, fnew, t, dt):
+        self.params.t, self.params.dt = float(t), float(dt)
+        self._ok(self.gen.lbm_gen_step(self._p(f), self._p(fnew), ctypes.byref(self.layout),
+                                       ctypes.byref(self.params), 0, self.stream), "one_time_step")
+
+    def f2m(self, f, m):
+        self._ok(self.gen.lbm_gen_f2m(self._p(f), ctypes.byref(self.layout), self._p(m),
+                                      ctypes.byref(self.mlayout), ctypes.byref(self.params), self.stream), "f2m")
+
+    def m2f(self, m, f):
+        self._ok(self.gen.lbm_gen_m2f(self._p(m), ctypes.byref(self.mlayout), self._p(f),
+                                      ctypes.byref(self.layout), ctypes.byref(self.params), self.stream), "m2f")
+
+    def equilibrium(self, m):
+        self._ok(self.gen.lbm_gen_equilibrium(self._p(m), ctypes.byref(self.mlayout), self.stream), "equilibrium")
+
+    # ------------------------------------------- ghost update (storage.py:306-367)
+    def update(self, f):
+        self._ok(self.core.lbm_ghost_fill_all(self._p(f), ctypes.byref(self.layout), ctypes.byref(self.vel),
+                                              self.stream), "update")
+
+    # ------------------------------------- boundary routines (boundary.py:430-895)
+    def _links(self, f, istore, iload0, rhs, sign, ncond, work, what):
+        self._ok(self.core.lbm_links_apply(self._p(f), ctypes.byref(self.layout), self._p(istore),
+                                           self._p(iload0), self._p(rhs), None, float(sign), int(ncond),
+                                           1 if work is not None else 0, self._p(work), self.stream), what)
+
+    def bounce_back(self, f, istore, iload0, rhs, ncond, *, work=None):
+        self._links(f, istore, iload0, rhs, 1.0, ncond, work, "bounce_back")
+
+    def anti_bounce_back(self, f, istore, iload0, rhs, ncond, *, work=None):
+        self._links(f, istore, iload0, rhs, -1.0, ncond, work, "anti_bounce_back")
+
+    def neumann(self, f, istore, iload0, ncond, *, work=None):
+        self._links(f, istore, iload0, None, 1.0, ncond, work, "neumann")
+
+    neumannx = neumanny = neumannz = neumann

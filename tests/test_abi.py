@@ -69,3 +69,26 @@ def test_gpu_path_fails_loudly_without_device():
         pytest.skip("a GPU is present")
     with pytest.raises(rt.HipUnavailable):
         rt.Runtime()
+
+
+def test_module_object_uses_pylbm_argument_names():
+    """pylbm's call_genfunction (pylbm/symbolic.py:288-299) passes exactly the names in
+    getfullargspec(func).args: the routines of the module object carry pylbm's
+    argument names (pylbm/algorithm/base.py:630-667, pylbm/boundary.py:358-376)."""
+    import inspect
+    from pylbm_amd.module import HipModule
+    expect = {
+        "one_time_step": ["f", "fnew", "t", "dt"],
+        "f2m": ["f", "m"],
+        "m2f": ["m", "f"],
+        "equilibrium": ["m"],
+        "update": ["f"],
+        "bounce_back": ["f", "istore", "iload0", "rhs", "ncond"],
+        "anti_bounce_back": ["f", "istore", "iload0", "rhs", "ncond"],
+        "neumann": ["f", "istore", "iload0", "ncond"],
+        "neumannx": ["f", "istore", "iload0", "ncond"],
+        "neumanny": ["f", "istore", "iload0", "ncond"],
+        "neumannz": ["f", "istore", "iload0", "ncond"],
+    }
+    for name, args in expect.items():
+        assert inspect.getfullargspec(getattr(HipModule, name)).args == ["self"] + args, name

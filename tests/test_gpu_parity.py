@@ -355,3 +355,53 @@ def test_distributions_init_equals_moments_init(hip):
     b.run(20)
     for k in range(a.ns):
         assert np.array_equal(a.F[k], b.F[k]), k
+
+
+def _call_genfunction(func, args):
+    """pylbm's call convention (pylbm/symbolic.py:288-299): arguments by name."""
+    import inspect
+    names = [n for n in inspect.getfullargspec(func).args if n != "self"]
+    return func(**{n: args[n] for n in names})
+
+
+@pytest.mark.parametrize("make", [lambda: cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0)),
+                                  lambda: cases.lid_cavity_d2q9(24),
+                                  lambda: cases.d1q3_advection(16, walls=True)])
+def test_hip_module_drives_pylbm_step_loop(hip, make):
+    """The generator module object (pylbm_amd.module.HipModule) driven the way pylbm's
+    Simulation.one_time_step drives its generated module: update, each link routine
+    with BoundaryMethod's argument names, one_time_step with BaseAlgorithm's; the
+    interior equals Simulation.one_time_step's bit for bit."""
+    import inspect
+    from pylbm_amd.module import HipModule
+    d = make()
+    a = Simulation(d)
+    b = Simulation(d, initialize=False)
+    b.fold_links = False
+    b._initialize()
+    mod = HipModule.from_simulation(b)
+    work = {id(dl): b.rt.malloc(8 * max(1, dl.n)) for dl in b._dev_links}
+    t = 0.0
+    for _ in range(12):
+        F, Fn = b.container.F, b.container.Fnew
+        args = {"f": F.ptr, "fnew": Fn.ptr, "t": t, "dt": b.dt, "m": b.container.m.ptr}
+        _call_genfunction(mod.update, args)
+        for dl in b._dev_links:
+            if isinstance(dl.method, str):
+                continue   # the ghost copy list of the restricted fill: update wraps every slot
+            bargs = {"f": F.ptr, "istore": dl.d_store, "iload0": dl.d_load, "rhs": dl.d_rhs, "ncond": dl.n}
+            routine = getattr(mod, dl.method.kind.name)
+            if dl.conflict:
+                # a list that reads its own stores runs resolved, gather before scatter
+                names = [n for n in inspect.getfullargspec(routine).args if n != "self"]
+                routine(**{n: bargs[n] for n in names}, work=work[id(dl)])
+            else:
+                _call_genfunction(routine, bargs)
+        _call_genfunction(mod.one_time_step, args)
+        b.container.F, b.container.Fnew = Fn, F
+        t += b.dt
+        a.one_time_step()
+    for k in range(a.ns):
+        assert np.array_equal(a.F[k], b.F[k]), k
+    for p in work.values():
+        b.rt.free(p)
