@@ -307,7 +307,6 @@ class DeviceLinks:
         self.layout = layout
         st, ld, sign, rhs, self.conflict, self.order = host_lists(method, layout, wshift, flat)
         self.n = st.size
-        self.rhs_host = rhs
         # host copies of the final lists, for merge_flags and link folding (dropped after use)
         self.h_store, self.h_load = st, ld
         self.h_sign = sign

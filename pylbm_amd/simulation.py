@@ -246,7 +246,7 @@ class Simulation:
         for ls, flag in zip(self._link_sets, flags):
             ls.merge_prev = flag if self.merge_links else 0
         for d in self._dev_links:
-            d.h_store = d.h_load = None
+            d.h_store = d.h_load = d.h_sign = d.h_rhs = None
         self._need_init = False
 
     def _fold_links(self):
