@@ -238,9 +238,13 @@ def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), 
     rt.record(e1)
     lbm_step_us = 1e3 * rt.elapsed_ms(e0, e1) / 400
     by_h = {}
-    for hz in horizons:
+
+    def judged(hz, je):
+        """Per held-out state, over hz steps with a judge every je intervals: plain and
+        judged-surrogate wall times, speedup, velocity rel-L2, fall-back, drifts."""
         nint = hz // K
-        evals = []
+        out = []
+        run.judge_every = je
         for seed in eval_seeds:
             s0 = lat.initial_state(seed)
             rt.h2d(run.state, s0)
@@ -262,10 +266,16 @@ def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), 
             rt.sync()
             t_sur = time.perf_counter() - t0
             err = run.rel_l2(run.state, ref)
-            evals.append({"seed": seed, "lbm_s": round(t_lbm, 5), "surrogate_s": round(t_sur, 5),
-                          "speedup": round(t_lbm / t_sur, 3), "rel_l2_velocity": float("%.3e" % err),
-                          "fallback_at_interval": run.fallback_at,
-                          "judge_drifts": [float("%.2e" % j) for j in run.judges]})
+            out.append({"seed": seed, "lbm_s": round(t_lbm, 5), "surrogate_s": round(t_sur, 5),
+                        "speedup": round(t_lbm / t_sur, 3), "rel_l2_velocity": float("%.3e" % err),
+                        "fallback_at_interval": run.fallback_at,
+                        "judge_drifts": [float("%.2e" % j) for j in run.judges]})
+        run.judge_every = judge_every
+        return out
+
+    for hz in horizons:
+        nint = hz // K
+        evals = judged(hz, judge_every)
         # moments-only floor: the plain run restarted from its moments every K steps
         rt.h2d(work, lat.initial_state(eval_seeds[0]))
         lat.restart(work)
@@ -282,6 +292,10 @@ def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), 
         by_h[str(hz)] = {"speedup": round(min(e["speedup"] for e in evals), 3),
                          "rel_l2_velocity": max(e["rel_l2_velocity"] for e in evals),
                          "moments_only_floor_rel_l2": float("%.3e" % floor), "eval": evals}
+    # sensitivity (not the configured schedule): the first horizon judged every 16 intervals
+    ev16 = judged(horizons[0], 16)
+    sens = {"horizon": horizons[0], "judge_every": 16, "speedup": round(min(e["speedup"] for e in ev16), 3),
+            "rel_l2_velocity": max(e["rel_l2_velocity"] for e in ev16), "eval": ev16}
     rt.free(ref)
     rt.free(work)
     run.free()
@@ -300,6 +314,7 @@ def _surrogate_leg(ntraj, steps, steps_per_traj, eval_seeds=(1000001, 1000002), 
                              "flops_per_step": flops,
                              "dtype": "%s x %s -> fp32" % (({"float16": "fp16", "bfloat16": "bf16"}[elem],) * 2)},
         "horizons": by_h,
+        "judge_every_16_sensitivity": sens,
         "train": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()},
     }
 

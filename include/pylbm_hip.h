@@ -213,6 +213,11 @@ int lbm_gen_m2f(const double* d_m, const lbm_layout* mlayout, void* d_f,
                 const lbm_layout* flayout, const lbm_params* params, void* stream);
 /* equilibrium in place on FP64 moments. pylbm/algorithm/base.py:381-402 */
 int lbm_gen_equilibrium(double* d_m, const lbm_layout* mlayout, void* stream);
+/* lbm_gen_equilibrium then lbm_gen_m2f in one pass (the moments stay in registers;
+ * only the conserved rows of d_m are read, none is written): the moment
+ * initialisation of pylbm/simulation.py:314-316 */
+int lbm_gen_eq_m2f(const double* d_m, const lbm_layout* mlayout, void* d_f,
+                   const lbm_layout* flayout, const lbm_params* params, void* stream);
 
 /*
  * nsteps complete time steps (ghost fill, link lists in order, fused step,

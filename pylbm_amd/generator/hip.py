@@ -526,6 +526,34 @@ equilibrium_kernel(double* __restrict__ mo, lbm_layout ml) {
 %(eq_stores)s
 }
 
+// A value the compiler must treat as opaque: an equilibrium moment used as if it
+// had been stored and loaded again, so that none of its products is contracted
+// into an FMA of m2f's sums (which would round differently from the two-kernel path).
+__device__ __forceinline__ double stored_moment(double x) {
+  asm("" : "+v"(x));
+  return x;
+}
+
+// equilibrium from the conserved moments followed by m2f, the moments kept in
+// registers: the same expressions as equilibrium_kernel then m2f_kernel, so the
+// same populations (the non-conserved rows of mo are not written)
+template <typename S>
+__global__ void __launch_bounds__(256)
+eq_m2f_kernel(const double* __restrict__ mo, lbm_layout ml, S* __restrict__ f, lbm_layout fl,
+              const StepArgs<double> a) {
+  using C = double;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= fl.n[2]) return;
+  const int64_t m_ = blockIdx.y, o_ = blockIdx.z;
+  const int64_t cf = cell_offset(fl, o_, m_, i);
+  const int64_t cm = cell_offset(ml, o_, m_, i);
+%(eqm2f_loads)s
+%(eq_body)s
+%(eqm2f_opaque)s
+%(m2f_full)s
+%(m2f_stores)s
+}
+
 template <typename C>
 static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   int rc = check_layout(L);
@@ -626,6 +654,19 @@ static int m2f_launch(const double* mi, const lbm_layout* ml, void* f, const lbm
   return 0;
 }
 
+template <typename S>
+static int eq_m2f_launch(const double* mi, const lbm_layout* ml, void* f, const lbm_layout* fl,
+                         const lbm_params* p, hipStream_t s) {
+  StepArgs<double> a;
+  int rc = make_args<double>(fl, p, &a);
+  if (rc) return rc;
+  const unsigned bx = block_x(fl->n[2]);
+  dim3 grid((unsigned)((fl->n[2] + bx - 1) / bx), (unsigned)fl->n[1], (unsigned)fl->n[0]);
+  eq_m2f_kernel<S><<<grid, bx, 0, s>>>(mi, *ml, (S*)f, *fl, a);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
 static int same_extents(const lbm_layout* a, const lbm_layout* b) {
   LBM_REQUIRE(a && b, "layout is NULL");
   LBM_REQUIRE(b->storage == LBM_F64, "moment layout must be FP64");
@@ -666,6 +707,15 @@ int lbm_gen_m2f(const double* d_m, const lbm_layout* ml, void* d_f, const lbm_la
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   LBM_DISPATCH_STORAGE(fl->storage, m2f_launch, d_m, ml, d_f, fl, params, s);
+}
+
+int lbm_gen_eq_m2f(const double* d_m, const lbm_layout* ml, void* d_f, const lbm_layout* fl,
+                   const lbm_params* params, void* stream) {
+  int rc = same_extents(fl, ml);
+  if (rc) return rc;
+  LBM_REQUIRE(ml->storage == LBM_F64 && ml->ns == NS, "eq_m2f needs FP64 moments of ns=%%d", NS);
+  hipStream_t s = (hipStream_t)stream;
+  LBM_DISPATCH_STORAGE(fl->storage, eq_m2f_launch, d_m, ml, d_f, fl, params, s);
 }
 
 int lbm_gen_equilibrium(double* d_m, const lbm_layout* ml, void* stream) {
@@ -836,6 +886,10 @@ def generate_source(spec: KernelSpec) -> str:
     eq_body = _equilibrium_lines(spec)
     eq_stores = ["mo[%d * ml.pop_stride + cm] = m%d;" % (i, i) for i in range(spec.nconsm, ns)]
     eq_loads += ["(void)m%d;" % i for i in range(spec.nconsm, ns)]
+    # the fused kernel loads only the conserved rows; the equilibrium assigns the others
+    eqm2f_loads = (["C m%d = mo[%d * ml.pop_stride + cm];" % (i, i) for i in range(spec.nconsm)]
+                   + ["C m%d;" % i for i in range(spec.nconsm, ns)])
+    eqm2f_opaque = ["m%d = stored_moment(m%d);" % (i, i) for i in range(spec.nconsm, ns)]
 
     vel_txt = ", ".join("{%d, %d, %d}" % v for v in vel)
     return _TEMPLATE % dict(
@@ -849,4 +903,5 @@ def generate_source(spec: KernelSpec) -> str:
         f2m_loads=_indent(f2m_loads), f2m_full=_indent(f2m_full), f2m_stores=_indent(f2m_stores),
         m2f_loads=_indent(m2f_loads), m2f_full=_indent(m2f_full), m2f_stores=_indent(m2f_stores),
         eq_loads=_indent(eq_loads), eq_body=_indent(eq_body), eq_stores=_indent(eq_stores),
+        eqm2f_loads=_indent(eqm2f_loads), eqm2f_opaque=_indent(eqm2f_opaque),
     )

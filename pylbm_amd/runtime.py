@@ -136,6 +136,8 @@ GEN_SIGNATURES = {
     "lbm_gen_m2f": ([_P, ctypes.POINTER(LbmLayout), _P, ctypes.POINTER(LbmLayout),
                      ctypes.POINTER(LbmParams), _P], _I),
     "lbm_gen_equilibrium": ([_P, ctypes.POINTER(LbmLayout), _P], _I),
+    "lbm_gen_eq_m2f": ([_P, ctypes.POINTER(LbmLayout), _P, ctypes.POINTER(LbmLayout),
+                        ctypes.POINTER(LbmParams), _P], _I),
     "lbm_gen_run": ([_P, _P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmVelocities),
                      ctypes.POINTER(LbmParams), ctypes.POINTER(LbmLinkSet), _I, _I64, _I, _I,
                      ctypes.POINTER(_I), _P], _I),

@@ -459,7 +459,8 @@ class Simulation:
                        self.rt.stream), "ghost fill")
         devs = self._dev_links
         if len(devs) > len(self.bc.methods):
-            devs[0].apply(f.ptr)
+            if not devs[0].fused:   # the ghost copy list, unless folded into the step
+                devs[0].apply(f.ptr)
             devs = devs[1:]
         for method, dev in zip(self.bc.methods, devs):
             self._update_feq(method, dev)

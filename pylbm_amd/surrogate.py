@@ -346,7 +346,8 @@ class LatticeState:
         self._bound = (c.F.ptr, c.Fnew.ptr)
 
     def restart(self, d_state):
-        """Populations from a state: conserved moments -> equilibrium -> m2f (pylbm moment init)."""
+        """Populations from a state: conserved moments -> equilibrium -> m2f (pylbm moment init),
+        the last two in one kernel."""
         sim = self.sim
         c = sim.container
         if getattr(self, "_bound", None) != (c.F.ptr, c.Fnew.ptr):
@@ -354,9 +355,8 @@ class LatticeState:
         st, lib, mod = self.rt.stream, self.lib, sim.module
         self.rt.check(lib.lbm_sur_state_to_moments(ctypes.c_void_p(d_state), self.H, self.W, self._m_lay, self.irho,
                                                    self.iqx, self.iqy, self._m_ptr, st), "lbm_sur_state_to_moments")
-        self.rt.check(mod.lbm_gen_equilibrium(self._m_ptr, self._m_lay, st), "equilibrium")
-        self.rt.check(mod.lbm_gen_m2f(self._m_ptr, self._m_lay, ctypes.c_void_p(c.F.ptr), self._f_lay, self._params,
-                                      st), "m2f")
+        self.rt.check(mod.lbm_gen_eq_m2f(self._m_ptr, self._m_lay, ctypes.c_void_p(c.F.ptr), self._f_lay,
+                                         self._params, st), "eq_m2f")
         # the fold step writes every cell of Fnew and reads no halo of F
         sim._update_m = True
 

@@ -405,3 +405,42 @@ def test_hip_module_drives_pylbm_step_loop(hip, make):
         assert np.array_equal(a.F[k], b.F[k]), k
     for p in work.values():
         b.rt.free(p)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32", "float32-shifted"])
+@pytest.mark.parametrize("make", [lambda: cases.lid_cavity_d2q9(24), lambda: cases.taylor_green_d3q19(12),
+                                  lambda: cases.karman_d2q9(64, 32, radius=4.0, center=(16.0, 16.0))])
+def test_eq_m2f_equals_equilibrium_then_m2f(hip, make, dtype):
+    """lbm_gen_eq_m2f (equilibrium of the conserved moments and m2f in one pass) stores
+    the same populations as the generated equilibrium and m2f routines called in turn
+    (pylbm's moment initialisation, pylbm/simulation.py:257-320), bit for bit, and
+    reads nothing but the conserved rows of the moment array."""
+    import ctypes
+    sim = Simulation(make(), dtype=dtype)
+    rt, mod = sim.rt, sim.module
+    m, F, Fn = sim.container.m, sim.container.F, sim.container.Fnew
+    cons = sorted(set(int(i) for i in m.consm.values()))
+    raw = m._raw_all()
+    rng = np.random.default_rng(7)
+    for k in range(m.ns):
+        if k not in cons:
+            raw[k] = rng.standard_normal(raw[k].shape)   # must not reach the populations
+        else:
+            raw[k] *= 1.0 + 1e-3 * rng.standard_normal(raw[k].shape)
+    m._write_all(raw)
+    st = rt.stream
+    p = ctypes.c_void_p
+    rt.check(mod.lbm_gen_eq_m2f(p(m.ptr), ctypes.byref(m.layout), p(F.ptr), ctypes.byref(F.layout),
+                                ctypes.byref(sim.params), st), "eq_m2f")
+    rt.check(mod.lbm_gen_equilibrium(p(m.ptr), ctypes.byref(m.layout), st), "equilibrium")
+    rt.check(mod.lbm_gen_m2f(p(m.ptr), ctypes.byref(m.layout), p(Fn.ptr), ctypes.byref(Fn.layout),
+                             ctypes.byref(sim.params), st), "m2f")
+    rt.sync()
+    a, b = F.fields_raw(), Fn.fields_raw()
+    sl = (slice(None),) + tuple(slice(h, -h) if h > 0 else slice(None) for h in F.vmax)
+    assert np.array_equal(a[sl].view(np.uint8), b[sl].view(np.uint8))
+    # a moment layout of the wrong row count is refused, not read out of bounds
+    bad = type(m.layout).from_buffer_copy(m.layout)
+    bad.ns = m.ns - 1
+    assert mod.lbm_gen_eq_m2f(p(m.ptr), ctypes.byref(bad), p(F.ptr), ctypes.byref(F.layout),
+                              ctypes.byref(sim.params), st) != 0
