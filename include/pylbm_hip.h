@@ -146,6 +146,23 @@ int lbm_links_apply(void* d_f, const lbm_layout* layout, const uint32_t* d_store
                     double sign, int64_t n, int two_phase, void* d_work, void* stream);
 
 /*
+ * One interpolated (Bouzidi) link list,
+ *   f[store[i]] = ca[i] * f[load0[i]] + cb[i] * f[load1[i]] + rhs[i],
+ * with ca, cb and rhs in the compute type (double for FP64 storage, float
+ * otherwise; rhs may be NULL).  Replaces the generated loops Bouzidi_bounce_back
+ * (pylbm/boundary.py:506-623: ca = s, cb = 1 - s, loads from a copy of f taken
+ * before the loop) and Bouzidi_anti_bounce_back (687-760: ca = -s, cb = 1 - s),
+ * where for a wall at distance q the host sets s = 2q, load1 = (ksym, x + 2 v_k)
+ * when q < 0.5, and s = 1/(2q), load1 = (k, x + v_k) otherwise.  two_phase != 0
+ * gathers every value into d_work (n values of the compute type) before any
+ * store: the copy semantics, for lists that load slots they store.
+ */
+int lbm_links_apply_bouzidi(void* d_f, const lbm_layout* layout, const uint32_t* d_store,
+                            const uint32_t* d_load0, const uint32_t* d_load1, const void* d_ca,
+                            const void* d_cb, const void* d_rhs, int64_t n, int two_phase, void* d_work,
+                            void* stream);
+
+/*
  * FP64 compensated (Neumaier) sums of every population over the interior.
  * d_partial needs 2*ns*(nblocks+1) doubles (nblocks from lbm_reduce_blocks),
  * h_sum/h_comp receive ns values each; the conserved sums are
@@ -238,6 +255,12 @@ typedef struct lbm_link_set {
    * lists store: it may run in the same launch as them (set by the host) */
   int32_t merge_prev;
   void* d_work;
+  /* Interpolated (Bouzidi) list when d_load1 is not NULL:
+   * f[store] = d_ca[i] f[load] + d_cb[i] f[load1] + rhs (compute-type arrays, see
+   * lbm_links_apply_bouzidi); it always runs alone (merge_prev ignored, d_sign unused) */
+  const uint32_t* d_load1;
+  const void* d_ca;
+  const void* d_cb;
 } lbm_link_set;
 
 /*

@@ -12,12 +12,23 @@ Follows pylbm/boundary.py:
     value function on moments, equilibrium, m2f                    boundary.py:238-305
   * set_rhs: BB feq[k] - feq[ksym]; ABB feq[k] + feq[ksym]         boundary.py:421-427, 637-643
   * loops f[istore] = +-f[iload] + rhs in link order (sequential)  boundary.py:430-464, 646-684, 787-818
+  * Bouzidi: s = 2q, iload1 = (ksym, x_out + 2 v_k) for q < 0.5; s = 1/(2q),
+    iload1 = (k, x_out + v_k) otherwise (iload0 as bounce-back); loop
+    f[istore] = +-s f[iload0] + (1 - s) f[iload1] + rhs, reading a copy of f
+    taken before the loop for bounce-back, f itself for anti-bounce-back
+                                                   boundary.py:506-549, 571-623, 707-760
 
 Pinned by: invariants checked in tests/test_oracle_boundary.py -- every
 store slot (k, x_out) has x_out + v_k fluid and flag[sym k][x_out + v_k] ==
 label, the link count equals the brute-force count of (fluid x, k) with a
 labelled neighbour, the ordering is (label, method, scheme, velocity,
 C-order) -- and bounce-back on a resting wall conserving mass per link pair.
+Bouzidi (tests/test_oracle_bouzidi.py): with every wall at q = 1/2 it equals
+bounce-back / anti-bounce-back bit for bit (s = 1); a force-driven channel
+whose walls cut the links at q = 0.3 reaches the Poiseuille profile of its true
+width, where plain bounce-back is off by the half-cell wall shift; per-link
+loads checked geometrically (x_out + v_k fluid, load1 one link further along
+v_k or the incoming population).
 """
 import numpy as np
 
@@ -30,6 +41,9 @@ NEUMANN = "neumann"
 NEUMANN_X = "neumannx"
 NEUMANN_Y = "neumanny"
 NEUMANN_Z = "neumannz"
+BZ_BB = "Bouzidi_bounce_back"
+BZ_ABB = "Bouzidi_anti_bounce_back"
+BOUNCE = (BB, ABB, BZ_BB, BZ_ABB)
 
 
 class LinkMethod:
@@ -91,9 +105,22 @@ def build_links(domain, stencil, bc_dico, method_kind):
         m = LinkMethod(kind, np.concatenate(store[kind], axis=1).astype(int),
                        np.concatenate(lab[kind]), np.concatenate(dist[kind]))
         k = m.istore[0]
-        if kind in (BB, ABB):
+        if kind in BOUNCE:
             m.iload = np.concatenate([ksym_all[k][None, :], m.istore[1:] + v_all[k].T])
-        else:
+        if kind in (BZ_BB, BZ_ABB):
+            q = m.distance
+            m.s = np.empty(q.size)
+            m.iload1 = np.empty_like(m.iload)
+            for i in range(q.size):
+                x = m.istore[1:, i]
+                v = v_all[k[i]]
+                if q[i] < 0.5:
+                    m.s[i] = 2.0 * q[i]
+                    m.iload1[:, i] = np.concatenate([[ksym_all[k[i]]], x + 2 * v])
+                else:
+                    m.s[i] = 0.5 / q[i]
+                    m.iload1[:, i] = np.concatenate([[k[i]], x + v])
+        elif kind not in BOUNCE:
             sh = v_all[k].T.copy()
             if kind != NEUMANN:
                 axis = {NEUMANN_X: 0, NEUMANN_Y: 1, NEUMANN_Z: 2}[kind]
@@ -105,8 +132,24 @@ def build_links(domain, stencil, bc_dico, method_kind):
     return out
 
 
+def _apply_bouzidi(f, method):
+    """f[istore] = a s f[iload0] + (1 - s) f[iload1] + rhs in link order, a = +1 reading a
+    copy of f taken before the loop (bounce-back), a = -1 reading f (anti-bounce-back)."""
+    a = 1.0 if method.kind == BZ_BB else -1.0
+    flat = f.reshape(-1)
+    src = flat.copy() if method.kind == BZ_BB else flat
+    st = np.ravel_multi_index(tuple(method.istore), f.shape)
+    l0 = np.ravel_multi_index(tuple(method.iload), f.shape)
+    l1 = np.ravel_multi_index(tuple(method.iload1), f.shape)
+    for i in range(st.size):
+        flat[st[i]] = a * method.s[i] * src[l0[i]] + (1.0 - method.s[i]) * src[l1[i]] + method.rhs[i]
+
+
 def apply_links(f, method):
     """f[istore] = a * f[iload] + rhs, in link order (reference loop semantics)."""
+    if method.kind in (BZ_BB, BZ_ABB):
+        _apply_bouzidi(f, method)
+        return
     s = tuple(method.istore)
     l = tuple(method.iload)
     sign = -1.0 if method.kind == ABB else 1.0

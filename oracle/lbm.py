@@ -36,12 +36,14 @@ import math
 
 import numpy as np
 
-from .boundary import ABB, BB, NEUMANN, NEUMANN_X, NEUMANN_Y, NEUMANN_Z, apply_links, build_links
+from .boundary import (ABB, BB, BZ_ABB, BZ_BB, NEUMANN, NEUMANN_X, NEUMANN_Y, NEUMANN_Z, apply_links,
+                       build_links)
 from .domain import Domain
 from .scheme import Scheme
 
 _KINDS = {"BounceBack": BB, "AntiBounceBack": ABB, "Neumann": NEUMANN,
-          "NeumannX": NEUMANN_X, "NeumannY": NEUMANN_Y, "NeumannZ": NEUMANN_Z}
+          "NeumannX": NEUMANN_X, "NeumannY": NEUMANN_Y, "NeumannZ": NEUMANN_Z,
+          "BouzidiBounceBack": BZ_BB, "BouzidiAntiBounceBack": BZ_ABB}
 
 
 def method_kind(cls):
@@ -157,10 +159,10 @@ class Simulation:
             f = self._m2f(m)
             meth.feq[:, idx[0]] = f.reshape(self.ns, n)
         k = meth.istore[0]
-        if meth.kind in (BB, ABB):
+        if meth.kind in (BB, ABB, BZ_BB, BZ_ABB):
             ksym = self.scheme.stencil.symmetric()[k]
             cols = np.arange(k.size)
-            sign = -1.0 if meth.kind == BB else 1.0
+            sign = -1.0 if meth.kind in (BB, BZ_BB) else 1.0
             meth.rhs = meth.feq[k, cols] + sign * meth.feq[ksym, cols]
 
     # ------------------------------------------------------------ one step

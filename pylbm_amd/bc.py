@@ -7,8 +7,13 @@ link-list rule of its reference counterpart:
 * ``AntiBounceBack``  f[k, x_out] = -f[ksym, x_out + v_k] + (feq_k + feq_ksym)
 * ``Neumann``         f[k, x_out] = f[k, x_out + v_k]
 * ``NeumannX/Y/Z``    f[k, x_out] = f[k, x_out + v_k projected on one axis]
+* ``BouzidiBounceBack``      f[k, x_out] = s f[ksym, x_out + v_k] + (1 - s) f[load1] + (feq_k - feq_ksym)
+* ``BouzidiAntiBounceBack``  f[k, x_out] = -s f[ksym, x_out + v_k] + (1 - s) f[load1] + (feq_k + feq_ksym)
+  with, for the wall distance q of the link, s = 2q and load1 = (ksym, x_out + 2 v_k)
+  when q < 0.5, s = 1/(2q) and load1 = (k, x_out + v_k) otherwise.
 
-The device work is done by ``lbm_links_apply`` (include/pylbm_hip.h).
+The device work is done by ``lbm_links_apply`` and ``lbm_links_apply_bouzidi``
+(include/pylbm_hip.h).
 """
 
 __all__ = ["BounceBack", "AntiBounceBack", "Neumann", "NeumannX", "NeumannY", "NeumannZ",
@@ -22,6 +27,8 @@ class BoundaryMethodKind:
     symmetric_load = True  # load population is ksym (bounce-back family) or k (Neumann)
     rhs_sign = None       # rhs = feq[k] + rhs_sign * feq[ksym]; None: no rhs
     shift_axis = None     # Neumann variants: only this component of v_k is applied
+    bouzidi = False       # interpolated: two loads weighted by the wall distance
+    reads_copy = False    # loads come from a copy of f taken before the list runs
 
 
 class BounceBack(BoundaryMethodKind):
@@ -57,12 +64,19 @@ class NeumannZ(Neumann):
 
 
 class BouzidiBounceBack(BoundaryMethodKind):
-    """Interpolated bounce-back (pylbm/boundary.py:472-623): not provided by the hip generator."""
+    """Interpolated bounce-back (pylbm/boundary.py:472-623); its loop reads a copy of
+    f taken before it (boundary.py:545-549, 595-606)."""
     name = "Bouzidi_bounce_back"
-    unsupported = True
+    sign = 1.0
+    rhs_sign = -1.0
+    bouzidi = True
+    reads_copy = True
 
 
 class BouzidiAntiBounceBack(BoundaryMethodKind):
-    """Interpolated anti-bounce-back (pylbm/boundary.py:687-760): not provided by the hip generator."""
+    """Interpolated anti-bounce-back (pylbm/boundary.py:687-760); its loop reads f
+    itself (boundary.py:732-743)."""
     name = "Bouzidi_anti_bounce_back"
-    unsupported = True
+    sign = -1.0
+    rhs_sign = 1.0
+    bouzidi = True

@@ -58,6 +58,20 @@ class BoundaryMethod:
     def set_iload(self):
         k = self.istore[0]
         v = self.stencil.get_all_velocities()
+        if self.kind.bouzidi:
+            # pylbm/boundary.py:506-535: a wall nearer than half a link (q < 0.5)
+            # interpolates between the fluid cell and the one behind it (s = 2q), a
+            # farther one between the bounced and the incoming population (s = 1/(2q))
+            ksym = self.stencil.get_symmetric()[k][np.newaxis, :]
+            vk = v[k].T
+            near = np.asarray(self.distance) < 0.5
+            x1 = self.istore[1:] + vk
+            self.iload.append(np.concatenate([ksym, x1]))
+            self.iload.append(np.where(near[np.newaxis, :], np.concatenate([ksym, x1 + vk]),
+                                       np.concatenate([k[np.newaxis, :], x1])))
+            dist = np.asarray(self.distance, dtype=np.float64)
+            self.s = np.where(near, 2.0 * dist, 0.5 / np.where(near, 1.0, dist))
+            return
         if self.kind.symmetric_load:
             ksym = self.stencil.get_symmetric()[k][np.newaxis, :]
             self.iload.append(np.concatenate([ksym, self.istore[1:] + v[k].T]))
@@ -296,6 +310,60 @@ def host_lists(method, layout, wshift, flat=None):
     return st, ld, sign, rhs, conflict, order
 
 
+def bouzidi_lists(method, layout, wshift):
+    """The lists an interpolated (Bouzidi) method runs with:
+    (store, load0, load1, ca, cb, rhs, two_phase, index, shift) with flat element
+    indices of layout, f[store] = ca f[load0] + cb f[load1] + rhs.
+
+    Bouzidi bounce-back reads a copy of f taken before its loop (pylbm/boundary.py:
+    545-549), so every load sees the old value and the last store of a slot wins:
+    earlier duplicate stores are dropped and, when a load hits a stored slot, the
+    list runs two-phase (all values gathered before any store).  Bouzidi
+    anti-bounce-back reads f itself in loop order: a load of a slot that an
+    earlier link of the loop stored would need that link's new value, which this
+    form does not carry (NotImplementedError; it needs a fluid gap of one cell);
+    loads of slots stored later see the old value, as two-phase gives them.
+    index: link positions of the method kept, in run order (rhs[index] + shift is
+    the device rhs); shift: the rest-population term of deviation formats."""
+    rows_s = method._store_rows()
+    l0r, l1r = [(a.T if method._fixed else a) for a in method.iload[:2]]
+    dim = rows_s.shape[0] - 1
+    for rows in (rows_s, l0r, l1r):
+        for d in range(dim):
+            c = np.asarray(rows[1 + d])
+            if c.size and (c.min() < 0 or c.max() >= layout.n[3 - dim + d]):
+                raise ValueError("%s: a link slot lies outside the halo-inclusive grid" % method.name)
+    st, l0, l1 = _flat(rows_s, layout), _flat(l0r, layout), _flat(l1r, layout)
+    s = np.asarray(method.s, dtype=np.float64)
+    n = st.size
+    if not method.kind.reads_copy and n:
+        us, first = np.unique(st, return_index=True)
+        links = np.concatenate([np.arange(n), np.arange(n)])
+        loads = np.concatenate([l0, l1])
+        pos = np.minimum(np.searchsorted(us, loads), us.size - 1)
+        hit = us[pos] == loads
+        if np.any(first[pos[hit]] < links[hit]):
+            raise NotImplementedError("%s: a link loads a slot an earlier link of the list stores"
+                                      % method.name)
+    # the last store of each slot
+    _, last_rev = np.unique(st[::-1], return_index=True)
+    index = np.sort(n - 1 - last_rev)
+    st, l0, l1, s = st[index], l0[index], l1[index], s[index]
+    two = bool(np.isin(np.concatenate([l0, l1]), st).any())
+    if not two and st.size > 1:
+        order = np.argsort(st, kind="stable")
+        index, st, l0, l1, s = index[order], st[order], l0[order], l1[order], s[order]
+    ca = method.kind.sign * s
+    cb = 1.0 - s
+    shift = np.zeros(st.size)
+    if layout.storage in DEVIATION_FORMATS:
+        # deviations: d_s = ca d_0 + cb d_1 + (r + ca w_0 + cb w_1 - w_s)
+        shift = (ca * wshift[_population(l0, layout)] + cb * wshift[_population(l1, layout)]
+                 - wshift[_population(st, layout)])
+    rhs = np.asarray(method.rhs, dtype=np.float64)[index] + shift
+    return st, l0, l1, ca, cb, rhs, two, index, shift
+
+
 class DeviceLinks:
     """Device arrays of one method for a given population layout."""
 
@@ -305,6 +373,11 @@ class DeviceLinks:
         self.rt = rt
         self.method = method
         self.layout = layout
+        self.bouzidi = flat is None and bool(getattr(method.kind, "bouzidi", False))
+        self.d_load1 = self.d_ca = self.d_cb = None
+        if self.bouzidi:
+            self._init_bouzidi(method, layout, wshift)
+            return
         st, ld, sign, rhs, self.conflict, self.order = host_lists(method, layout, wshift, flat)
         self.n = st.size
         # host copies of the final lists, for merge_flags and link folding (dropped after use)
@@ -323,6 +396,24 @@ class DeviceLinks:
         if self.d_work:
             self._ptrs.append(self.d_work)
 
+    def _init_bouzidi(self, method, layout, wshift):
+        st, l0, l1, ca, cb, rhs, self.conflict, self.order, self._shift = bouzidi_lists(method, layout, wshift)
+        self.n = st.size
+        self.h_store, self.h_load = st, np.concatenate([l0, l1])
+        self.h_sign, self.h_rhs = None, None
+        ctype = np.float64 if layout.storage == LBM_F64 else np.float32
+        self._ptrs = []
+        self.d_store = self._up(st.astype(np.uint32))
+        self.d_load = self._up(l0.astype(np.uint32))
+        self.d_load1 = self._up(l1.astype(np.uint32))
+        self.d_ca = self._up(ca.astype(ctype))
+        self.d_cb = self._up(cb.astype(ctype))
+        self.d_rhs = self._up(rhs.astype(ctype)) if np.any(rhs != 0) else None
+        self.sign, self.d_sign = 1.0, None
+        self.d_work = self.rt.malloc(max(1, self.n) * np.dtype(ctype).itemsize) if self.conflict else None
+        if self.d_work:
+            self._ptrs.append(self.d_work)
+
     def _up(self, a):
         p = self.rt.upload(a)
         self._ptrs.append(p)
@@ -330,6 +421,14 @@ class DeviceLinks:
 
     def update_rhs(self, rhs, wshift):
         """Re-upload rhs (time-dependent boundary values)."""
+        if self.bouzidi:
+            ctype = np.float64 if self.layout.storage == LBM_F64 else np.float32
+            dev = (np.asarray(rhs, dtype=np.float64)[self.order] + self._shift).astype(ctype)
+            if self.d_rhs is None:
+                self.d_rhs = self._up(dev)
+            else:
+                self.rt.h2d(self.d_rhs, dev)
+            return
         if self.conflict:
             raise NotImplementedError("time-dependent values on a self-overlapping link list")
         if self.layout.storage in DEVIATION_FORMATS:
@@ -355,10 +454,18 @@ class DeviceLinks:
         s.two_phase = 1 if self.conflict else 0
         s.merge_prev = 0
         s.d_work = self.d_work
+        s.d_load1, s.d_ca, s.d_cb = self.d_load1, self.d_ca, self.d_cb
         return s
 
     def apply(self, d_f):
         rt = self.rt
+        if self.bouzidi:
+            p = ctypes.c_void_p
+            rt.check(rt.lib.lbm_links_apply_bouzidi(p(d_f), ctypes.byref(self.layout), p(self.d_store),
+                                                    p(self.d_load), p(self.d_load1), p(self.d_ca), p(self.d_cb),
+                                                    p(self.d_rhs), self.n, 1 if self.conflict else 0,
+                                                    p(self.d_work), rt.stream), self.method.name)
+            return
         rt.check(rt.lib.lbm_links_apply(ctypes.c_void_p(d_f), ctypes.byref(self.layout),
                                         ctypes.c_void_p(self.d_store), ctypes.c_void_p(self.d_load),
                                         ctypes.c_void_p(self.d_rhs), ctypes.c_void_p(self.d_sign),

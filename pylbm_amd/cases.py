@@ -24,14 +24,14 @@ import numpy as np
 import sympy as sp
 
 from . import bc
-from .elements import Circle, Sphere
+from .elements import Circle, Parallelogram, Sphere
 
 X, Y, Z = sp.symbols("X, Y, Z")
 RHO, QX, QY, QZ = sp.symbols("rho, qx, qy, qz")
 LA = sp.symbols("lambda", constants=True)
 
 __all__ = ["d2q9_mrt", "d3q19_mrt", "lid_cavity_d2q9", "karman_d2q9", "taylor_green_d3q19",
-           "sphere_packing_d3q19", "decaying_d2q9", "RHO", "QX", "QY", "QZ"]
+           "sphere_packing_d3q19", "decaying_d2q9", "poiseuille_d2q9", "with_bouzidi", "RHO", "QX", "QY", "QZ"]
 
 
 def d2q9_mrt(s_e, s_eps, s_q, s_nu, rho0=1.0):
@@ -168,6 +168,39 @@ def karman_d2q9(nx=1024, ny=256, radius=20.0, center=(256.0, 128.0), u_in=0.05, 
         },
         "generator": generator,
     }
+
+
+def poiseuille_d2q9(nx=4, ny=12, y_low=1.1, y_high=10.6, g=1e-5, s_nu=1.0, method=bc.BounceBack,
+                    generator="hip"):
+    """Force-driven channel, periodic in x, between two solid slabs whose faces
+    y = y_low and y = y_high cut the links at fractional distances (label 0, method
+    `method`); body force g on qx.  Steady state: a parabola vanishing at the walls."""
+    sch = d2q9_mrt(1.2, 1.2, 1.2, s_nu)
+    sch["source_terms"] = {QX: g}
+    H = float(ny)
+    return {
+        "parameters": {LA: 1},
+        "box": {"x": [0.0, float(nx)], "y": [0.0, H], "label": [-1, -1, 0, 0]},
+        "elements": [Parallelogram((-2.0, -1.0), (nx + 4.0, 0.0), (0.0, y_low + 1.0), label=0),
+                     Parallelogram((-2.0, y_high), (nx + 4.0, 0.0), (0.0, H - y_high + 1.0), label=0)],
+        "space_step": 1.0,
+        "scheme_velocity": LA,
+        "schemes": [sch],
+        "init": {RHO: 1.0, QX: 0.0, QY: 0.0},
+        "boundary_conditions": {0: {"method": {0: method}}},
+        "generator": generator,
+    }
+
+
+def with_bouzidi(d):
+    """The same case with its bounce-back / anti-bounce-back methods replaced by the
+    interpolated (Bouzidi) ones."""
+    swap = {bc.BounceBack: bc.BouzidiBounceBack, bc.AntiBounceBack: bc.BouzidiAntiBounceBack}
+    out = dict(d)
+    out["boundary_conditions"] = {
+        label: dict(spec, method={k: swap.get(cls, cls) for k, cls in spec["method"].items()})
+        for label, spec in d["boundary_conditions"].items()}
+    return out
 
 
 def taylor_green_d3q19(n=256, u0=0.04, nu=0.005, generator="hip"):

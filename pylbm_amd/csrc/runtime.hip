@@ -197,6 +197,48 @@ static int links_launch(void* d_f, const uint32_t* st, const uint32_t* ld, const
   return 0;
 }
 
+// Interpolated (Bouzidi) links, pylbm/boundary.py:608-623 and 745-760:
+// f[store] = ca f[load0] + cb f[load1] + rhs with one coefficient pair per link
+// (ca = s or -s, cb = 1 - s; the host picks the loads and s from the wall
+// distance).  work == nullptr: the value is stored at once (the list loads no slot
+// it stores); otherwise it is gathered into work and links_scatter_kernel stores it.
+template <typename S>
+__global__ void __launch_bounds__(256) bouzidi_kernel(S* __restrict__ f,
+                                                      const uint32_t* __restrict__ store,
+                                                      const uint32_t* __restrict__ load0,
+                                                      const uint32_t* __restrict__ load1,
+                                                      const typename Codec<S>::C* __restrict__ ca,
+                                                      const typename Codec<S>::C* __restrict__ cb,
+                                                      const typename Codec<S>::C* __restrict__ rhs,
+                                                      int64_t n, typename Codec<S>::C* __restrict__ work) {
+  using C = typename Codec<S>::C;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const C v0 = Codec<S>::load_dev(f, load0[t], (C)0);
+  const C v1 = Codec<S>::load_dev(f, load1[t], (C)0);
+  C r = ca[t] * v0 + cb[t] * v1;
+  if (rhs) r += rhs[t];
+  if (work)
+    work[t] = r;
+  else
+    Codec<S>::store(f, store[t], r, (C)0);
+}
+
+template <typename S>
+static int bouzidi_launch(void* d_f, const uint32_t* st, const uint32_t* l0, const uint32_t* l1,
+                          const void* ca, const void* cb, const void* rhs, int64_t n, int two_phase,
+                          void* work, hipStream_t s) {
+  using C = typename Codec<S>::C;
+  const int64_t blocks = (n + 255) / 256;
+  LBM_REQUIRE(!two_phase || work != nullptr, "two-phase link list needs a workspace");
+  bouzidi_kernel<S><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(
+      (S*)d_f, st, l0, l1, (const C*)ca, (const C*)cb, (const C*)rhs, n, two_phase ? (C*)work : nullptr);
+  if (two_phase)
+    links_scatter_kernel<S><<<dim3((unsigned)blocks), dim3(256), 0, s>>>((S*)d_f, st, n, (const C*)work);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
 // Several independent link lists (no list reads or writes a slot another one writes)
 // in one launch: thread t belongs to the list j with end[j - 1] <= t < end[j] and
 // applies that list's link t - end[j - 1] exactly as links_kernel does.
@@ -542,6 +584,20 @@ int lbm_links_apply(void* d_f, const lbm_layout* layout, const uint32_t* d_store
                        two_phase, d_work, s);
 }
 
+int lbm_links_apply_bouzidi(void* d_f, const lbm_layout* layout, const uint32_t* d_store,
+                            const uint32_t* d_load0, const uint32_t* d_load1, const void* d_ca,
+                            const void* d_cb, const void* d_rhs, int64_t n, int two_phase, void* d_work,
+                            void* stream) {
+  int rc = check_layout(layout);
+  if (rc) return rc;
+  if (n <= 0) return 0;
+  LBM_REQUIRE(n < ((int64_t)1 << 31) * 256, "link list too long");
+  LBM_REQUIRE(d_store && d_load0 && d_load1 && d_ca && d_cb, "Bouzidi index or coefficient arrays are NULL");
+  hipStream_t s = (hipStream_t)stream;
+  LBM_DISPATCH_STORAGE(layout->storage, bouzidi_launch, d_f, d_store, d_load0, d_load1, d_ca, d_cb, d_rhs,
+                       n, two_phase, d_work, s);
+}
+
 int lbm_links_apply_sets(void* d_f, const lbm_layout* layout, const lbm_link_set* sets, int nsets,
                          void* stream) {
   int rc = check_layout(layout);
@@ -552,8 +608,18 @@ int lbm_links_apply_sets(void* d_f, const lbm_layout* layout, const lbm_link_set
   while (j < nsets) {
     // a group: set j and the following sets marked merge_prev (at most 8, none two-phase)
     int k = j + 1;
+    if (sets[j].d_load1) {  // an interpolated list runs alone
+      const lbm_link_set& L = sets[j];
+      rc = lbm_links_apply_bouzidi(d_f, layout, L.d_store, L.d_load, L.d_load1, L.d_ca, L.d_cb, L.d_rhs, L.n,
+                                   L.two_phase, L.d_work, stream);
+      if (rc) return rc;
+      j = k;
+      continue;
+    }
     if (!sets[j].two_phase) {
-      while (k < nsets && k - j < LBM_MAX_LINK_GROUP && sets[k].merge_prev && !sets[k].two_phase) ++k;
+      while (k < nsets && k - j < LBM_MAX_LINK_GROUP && sets[k].merge_prev && !sets[k].two_phase &&
+             !sets[k].d_load1)
+        ++k;
     }
     if (k - j == 1) {
       const lbm_link_set& L = sets[j];

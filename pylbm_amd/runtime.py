@@ -57,7 +57,8 @@ class LbmParams(ctypes.Structure):
 class LbmLinkSet(ctypes.Structure):
     _fields_ = [("d_store", ctypes.c_void_p), ("d_load", ctypes.c_void_p), ("d_rhs", ctypes.c_void_p),
                 ("d_sign", ctypes.c_void_p), ("sign", ctypes.c_double), ("n", ctypes.c_int64),
-                ("two_phase", ctypes.c_int32), ("merge_prev", ctypes.c_int32), ("d_work", ctypes.c_void_p)]
+                ("two_phase", ctypes.c_int32), ("merge_prev", ctypes.c_int32), ("d_work", ctypes.c_void_p),
+                ("d_load1", ctypes.c_void_p), ("d_ca", ctypes.c_void_p), ("d_cb", ctypes.c_void_p)]
 
 
 _P = ctypes.c_void_p
@@ -93,6 +94,7 @@ CORE_SIGNATURES = {
     "lbm_ghost_fill": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmVelocities), _P], _I),
     "lbm_ghost_fill_all": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmVelocities), _P], _I),
     "lbm_links_apply": ([_P, ctypes.POINTER(LbmLayout), _P, _P, _P, _P, _D, _I64, _I, _P, _P], _I),
+    "lbm_links_apply_bouzidi": ([_P, ctypes.POINTER(LbmLayout), _P, _P, _P, _P, _P, _P, _I64, _I, _P, _P], _I),
     "lbm_links_apply_sets": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmLinkSet), _I, _P], _I),
     "lbm_reduce_blocks": ([ctypes.POINTER(LbmLayout), ctypes.POINTER(_I64)], _I),
     "lbm_reduce_populations": ([_P, ctypes.POINTER(LbmLayout), _P, _P, ctypes.POINTER(_D),

@@ -157,6 +157,10 @@ class Simulation:
         # slots any pull or Neumann-type list reads come from a host-computed copy list
         # (boundary.unwritten_ghost_reads); the full wrap (2) stays available
         self._ghost_mode = 1
+        if any(m.kind.bouzidi for m in self.bc.methods):
+            # an interpolated list may load any slot two cells from a wall: wrap
+            # every halo slot, as pylbm does (pylbm/storage.py:306-367)
+            self._ghost_mode = 2
         # consecutive independent link lists share one launch in run() (False: one
         # launch per list; the results are the same)
         self.merge_links = True
@@ -242,7 +246,7 @@ class Simulation:
         self._active_links = active
         self._link_sets = (rtm.LbmLinkSet * max(1, len(active)))(*[d.link_set() for d in active])
         # independent consecutive lists run in one launch (bit-identical results)
-        flags = merge_flags([(d.h_store, d.h_load, d.conflict) for d in active])
+        flags = merge_flags([(d.h_store, d.h_load, d.conflict or d.bouzidi) for d in active])
         for ls, flag in zip(self._link_sets, flags):
             ls.merge_prev = flag if self.merge_links else 0
         for d in self._dev_links:
@@ -280,6 +284,8 @@ class Simulation:
         devs = self._dev_links
         if not devs or any(not isinstance(d.method, str) and d.method._tbc for d in devs):
             return False
+        if any(d.bouzidi for d in devs):
+            return False   # an interpolated value has two sources
         L, n, h, nin = self._fold_layout()
         slot, org, sg, rh, has, bad = resolve_step([(d.h_store, d.h_load, d.h_sign, d.h_rhs) for d in devs])
         k, o, m, i = _unflat_fn(slot, L)

@@ -444,3 +444,52 @@ def test_eq_m2f_equals_equilibrium_then_m2f(hip, make, dtype):
     bad.ns = m.ns - 1
     assert mod.lbm_gen_eq_m2f(p(m.ptr), ctypes.byref(bad), p(F.ptr), ctypes.byref(F.layout),
                               ctypes.byref(sim.params), st) != 0
+
+
+_BOUZIDI = {
+    # walls at q = 0.7 (below) and 0.1 (above): both interpolation branches
+    "channel": lambda: cases.poiseuille_d2q9(y_low=0.8, y_high=10.6, method=bc.BouzidiBounceBack),
+    # a one-row gap: the list loads slots it stores, so it runs two-phase (pylbm's fcopy)
+    "gap": lambda: cases.poiseuille_d2q9(ny=5, y_low=1.3, y_high=2.4, method=bc.BouzidiBounceBack),
+    # circle (q in (0, 1)) and walls, Bouzidi anti-bounce-back inflow, NeumannX outflow
+    "karman": lambda: cases.with_bouzidi(cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0))),
+    "spheres": lambda: cases.with_bouzidi(cases.sphere_packing_d3q19(24, 16, 16, seed=5)),
+}
+
+
+@pytest.mark.parametrize("name", list(_BOUZIDI))
+def test_bouzidi_fp64_parity(hip, name):
+    """Interpolated bounce-back / anti-bounce-back (pylbm/boundary.py:472-760) against
+    the oracle's sequential loops, one_time_step and run() paths."""
+    d = _BOUZIDI[name]()
+    sim, ref = Simulation(d), Oracle(d)
+    bz = [dl for dl in sim._dev_links if dl.bouzidi]
+    assert bz
+    if name == "gap":
+        assert all(dl.conflict for dl in bz)
+    _march(sim, ref, 30, use_run=name in ("karman", "spheres"))
+    syms = [cases.RHO, cases.QX, cases.QY] + ([cases.QZ] if name == "spheres" else [])
+    _compare(sim, ref, syms, TOL64)
+
+
+@pytest.mark.parametrize("dtype", ["float32-shifted", "float16"])
+def test_bouzidi_deviation_formats(hip, dtype):
+    """Bouzidi lists with the rest-population shift in rhs
+    (d_s = ca d_0 + cb d_1 + r + ca w_0 + cb w_1 - w_s)."""
+    d = _BOUZIDI["karman"]()
+    sim, ref = Simulation(d, dtype=dtype), Oracle(d)
+    _march(sim, ref, 40, use_run=True)
+    _compare(sim, ref, [cases.RHO, cases.QX, cases.QY], 2e-6 if dtype == "float32-shifted" else 2e-3)
+
+
+@pytest.mark.parametrize("name", ["karman", "gap"])
+def test_bouzidi_run_equals_steps(hip, name):
+    """run() (the library's step loop with the Bouzidi link sets) equals repeated
+    one_time_step() (one lbm_links_apply_bouzidi call per list) bit for bit."""
+    d = _BOUZIDI[name]()
+    a, b = Simulation(d), Simulation(d)
+    a.run(25)
+    for _ in range(25):
+        b.one_time_step()
+    for k in range(a.ns):
+        assert np.array_equal(a.F[k], b.F[k]), k
