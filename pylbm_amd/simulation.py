@@ -157,10 +157,15 @@ class Simulation:
         # slots any pull or Neumann-type list reads come from a host-computed copy list
         # (boundary.unwritten_ghost_reads); the full wrap (2) stays available
         self._ghost_mode = 1
-        if any(m.kind.bouzidi for m in self.bc.methods):
-            # an interpolated list may load any slot two cells from a wall: wrap
-            # every halo slot, as pylbm does (pylbm/storage.py:306-367)
-            self._ghost_mode = 2
+        halo = np.asarray(self.domain.stencil.vmax)[:, None]
+        shape = np.asarray(self.domain.shape_halo)[:, None]
+        for m in self.bc.methods:
+            if m.kind.bouzidi and any(np.any((np.asarray(r[1:]) < halo) | (np.asarray(r[1:]) >= shape - halo))
+                                      for r in m.iload):
+                # an interpolated list loads a halo slot (its second load, one link
+                # beyond a fluid cell next to the box): wrap every halo slot, as pylbm
+                # does (pylbm/storage.py:306-367)
+                self._ghost_mode = 2
         # consecutive independent link lists share one launch in run() (False: one
         # launch per list; the results are the same)
         self.merge_links = True

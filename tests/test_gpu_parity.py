@@ -467,6 +467,8 @@ def test_bouzidi_fp64_parity(hip, name):
     assert bz
     if name == "gap":
         assert all(dl.conflict for dl in bz)
+    # every second load of the channel's walls is interior: the restricted ghost fill stays
+    assert sim._ghost_mode == (1 if name == "karman" else 2)
     _march(sim, ref, 30, use_run=name in ("karman", "spheres"))
     syms = [cases.RHO, cases.QX, cases.QY] + ([cases.QZ] if name == "spheres" else [])
     _compare(sim, ref, syms, TOL64)
