@@ -257,7 +257,7 @@ typedef struct lbm_link_set {
   void* d_work;
   /* Interpolated (Bouzidi) list when d_load1 is not NULL:
    * f[store] = d_ca[i] f[load] + d_cb[i] f[load1] + rhs (compute-type arrays, see
-   * lbm_links_apply_bouzidi); it always runs alone (merge_prev ignored, d_sign unused) */
+   * lbm_links_apply_bouzidi; d_sign unused); merge_prev as for the other lists */
   const uint32_t* d_load1;
   const void* d_ca;
   const void* d_cb;
@@ -267,7 +267,7 @@ typedef struct lbm_link_set {
  * The link lists of one time step in order (pylbm/simulation.py:385-390), in as
  * few launches as their merge_prev flags allow: a group of up to 8 consecutive
  * one-phase lists joined by merge_prev runs as one kernel, with each link
- * computed as lbm_links_apply computes it.
+ * computed as lbm_links_apply (or, d_load1 set, lbm_links_apply_bouzidi) computes it.
  */
 int lbm_links_apply_sets(void* d_f, const lbm_layout* layout, const lbm_link_set* sets, int nsets,
                          void* stream);

@@ -241,7 +241,8 @@ static int bouzidi_launch(void* d_f, const uint32_t* st, const uint32_t* l0, con
 
 // Several independent link lists (no list reads or writes a slot another one writes)
 // in one launch: thread t belongs to the list j with end[j - 1] <= t < end[j] and
-// applies that list's link t - end[j - 1] exactly as links_kernel does.
+// applies that list's link t - end[j - 1] exactly as links_kernel does (or, for a
+// one-phase interpolated list, load1[j] set, exactly as bouzidi_kernel does).
 constexpr int LBM_MAX_LINK_GROUP = 8;
 
 template <typename C>
@@ -250,6 +251,9 @@ struct LinkGroup {
   const uint32_t* load[LBM_MAX_LINK_GROUP];
   const C* rhs[LBM_MAX_LINK_GROUP];
   const int8_t* sgn[LBM_MAX_LINK_GROUP];
+  const uint32_t* load1[LBM_MAX_LINK_GROUP];
+  const C* ca[LBM_MAX_LINK_GROUP];
+  const C* cb[LBM_MAX_LINK_GROUP];
   float sign[LBM_MAX_LINK_GROUP];
   int64_t end[LBM_MAX_LINK_GROUP];
   int count;
@@ -264,9 +268,16 @@ __global__ void __launch_bounds__(256) links_group_kernel(S* __restrict__ f,
   int j = 0;
   while (t >= G.end[j]) ++j;
   const int64_t i = t - (j ? G.end[j - 1] : 0);
-  C v = Codec<S>::load_dev(f, G.load[j][i], (C)0);
-  const C a = G.sgn[j] ? (C)G.sgn[j][i] : (C)G.sign[j];
-  C r = a * v;
+  C r;
+  if (G.load1[j]) {
+    const C v0 = Codec<S>::load_dev(f, G.load[j][i], (C)0);
+    const C v1 = Codec<S>::load_dev(f, G.load1[j][i], (C)0);
+    r = G.ca[j][i] * v0 + G.cb[j][i] * v1;
+  } else {
+    C v = Codec<S>::load_dev(f, G.load[j][i], (C)0);
+    const C a = G.sgn[j] ? (C)G.sgn[j][i] : (C)G.sign[j];
+    r = a * v;
+  }
   if (G.rhs[j]) r += G.rhs[j][i];
   Codec<S>::store(f, G.store[j][i], r, (C)0);
 }
@@ -282,6 +293,9 @@ static int links_group_launch(void* d_f, const lbm_link_set* sets, int nsets, hi
     G.load[j] = sets[j].d_load;
     G.rhs[j] = (const C*)sets[j].d_rhs;
     G.sgn[j] = sets[j].d_sign;
+    G.load1[j] = sets[j].d_load1;
+    G.ca[j] = (const C*)sets[j].d_ca;
+    G.cb[j] = (const C*)sets[j].d_cb;
     G.sign[j] = (float)sets[j].sign;
     total += sets[j].n;
     G.end[j] = total;
@@ -608,26 +622,23 @@ int lbm_links_apply_sets(void* d_f, const lbm_layout* layout, const lbm_link_set
   while (j < nsets) {
     // a group: set j and the following sets marked merge_prev (at most 8, none two-phase)
     int k = j + 1;
-    if (sets[j].d_load1) {  // an interpolated list runs alone
+    if (!sets[j].two_phase) {
+      while (k < nsets && k - j < LBM_MAX_LINK_GROUP && sets[k].merge_prev && !sets[k].two_phase) ++k;
+    }
+    if (k - j == 1 && sets[j].d_load1) {
       const lbm_link_set& L = sets[j];
       rc = lbm_links_apply_bouzidi(d_f, layout, L.d_store, L.d_load, L.d_load1, L.d_ca, L.d_cb, L.d_rhs, L.n,
                                    L.two_phase, L.d_work, stream);
-      if (rc) return rc;
-      j = k;
-      continue;
-    }
-    if (!sets[j].two_phase) {
-      while (k < nsets && k - j < LBM_MAX_LINK_GROUP && sets[k].merge_prev && !sets[k].two_phase &&
-             !sets[k].d_load1)
-        ++k;
-    }
-    if (k - j == 1) {
+    } else if (k - j == 1) {
       const lbm_link_set& L = sets[j];
       rc = lbm_links_apply(d_f, layout, L.d_store, L.d_load, L.d_rhs, L.d_sign, L.sign, L.n, L.two_phase,
                            L.d_work, stream);
     } else {
-      for (int q = j; q < k; ++q)
+      for (int q = j; q < k; ++q) {
         LBM_REQUIRE(sets[q].n == 0 || (sets[q].d_store && sets[q].d_load), "store/load index arrays are NULL");
+        LBM_REQUIRE(sets[q].n == 0 || !sets[q].d_load1 || (sets[q].d_ca && sets[q].d_cb),
+                    "Bouzidi coefficient arrays are NULL");
+      }
       rc = links_group_dispatch(d_f, layout, sets + j, k - j, s);
     }
     if (rc) return rc;

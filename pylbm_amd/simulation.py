@@ -251,7 +251,7 @@ class Simulation:
         self._active_links = active
         self._link_sets = (rtm.LbmLinkSet * max(1, len(active)))(*[d.link_set() for d in active])
         # independent consecutive lists run in one launch (bit-identical results)
-        flags = merge_flags([(d.h_store, d.h_load, d.conflict or d.bouzidi) for d in active])
+        flags = merge_flags([(d.h_store, d.h_load, d.conflict) for d in active])
         for ls, flag in zip(self._link_sets, flags):
             ls.merge_prev = flag if self.merge_links else 0
         for d in self._dev_links:
