@@ -338,3 +338,67 @@ def test_resolve_step_equals_sequential_lists(name):
     untouched = np.ones(flat0.size, bool)
     untouched[slot] = False
     assert np.array_equal(flat0[untouched], flat1[untouched])
+
+
+_BZ = {
+    "gap-bb": lambda: cases.poiseuille_d2q9(ny=5, y_low=1.3, y_high=2.4, method=bc.BouzidiBounceBack),
+    # loads of slots stored later in the loop: the old values, as two-phase gives
+    "gap-abb": lambda: cases.poiseuille_d2q9(ny=5, y_low=1.3, y_high=2.4, method=bc.BouzidiAntiBounceBack),
+    "karman-bz": lambda: cases.with_bouzidi(cases.karman_d2q9(48, 24, radius=4.0, center=(12.0, 12.0))),
+    "spheres-bz": lambda: cases.with_bouzidi(cases.sphere_packing_d3q19(16, 12, 12, seed=3)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(_BZ))
+def test_bouzidi_lists_equal_sequential_loop(name):
+    """The device form of an interpolated list (boundary.bouzidi_lists: last store
+    wins, gather before scatter when loads hit stores, sorted otherwise), applied
+    to a flat array of the device layout, equals the oracle's loop in link order
+    (the copy of f for bounce-back, f itself for anti-bounce-back), bit for bit."""
+    from oracle.boundary import _apply_bouzidi
+    from pylbm_amd.boundary import _flat, bouzidi_lists
+    from pylbm_amd.runtime import LBM_F64
+    from pylbm_amd.storage import layout_elements, make_layout
+    d = _BZ[name]()
+    dom, pm = _product(d)
+    _, om = _oracle(d)
+    ns = int(dom.stencil.nv_ptr[-1])
+    shape = tuple(dom.shape_halo)
+    L = make_layout(ns, shape, list(dom.stencil.vmax), LBM_F64)
+    rng = np.random.default_rng(11)
+    F = rng.standard_normal((ns,) + shape)
+    every = np.array(np.unravel_index(np.arange(F.size), F.shape))
+    pos = _flat(every, L)
+    flat = np.full(layout_elements(L), np.nan)
+    flat[pos] = F.ravel()
+    checked = 0
+    for p, o in zip(pm, om):
+        if not p.kind.bouzidi:
+            continue
+        assert o.kind == p.name
+        rhs = rng.standard_normal(p.ncond)
+        p.rhs, o.rhs = rhs.copy(), rhs.copy()
+        st, l0, l1, ca, cb, r, two, index, shift = bouzidi_lists(p, L, np.zeros(ns))
+        vals = ca * flat[l0] + cb * flat[l1] + r      # every load before any store
+        flat[st] = vals
+        _apply_bouzidi(F, o)
+        assert np.array_equal(flat[pos], F.ravel())
+        if name.startswith("gap"):
+            assert two
+        checked += 1
+    assert checked
+
+
+def test_bouzidi_anti_bounce_back_refuses_earlier_store_loads():
+    """Bouzidi anti-bounce-back reads f in loop order: in a one-row gap whose upper
+    wall is the nearer one, the upper links (populations pointing down, later in the
+    velocity order) load slots of the lower solid row that the lower links stored
+    earlier, which the two-phase form cannot give."""
+    from pylbm_amd.boundary import bouzidi_lists
+    from pylbm_amd.runtime import LBM_F64
+    from pylbm_amd.storage import make_layout
+    d = cases.poiseuille_d2q9(ny=5, y_low=0.9, y_high=1.8, method=bc.BouzidiAntiBounceBack)
+    dom, pm = _product(d)
+    L = make_layout(int(dom.stencil.nv_ptr[-1]), tuple(dom.shape_halo), list(dom.stencil.vmax), LBM_F64)
+    with pytest.raises(NotImplementedError):
+        bouzidi_lists(pm[0], L, np.zeros(9))
