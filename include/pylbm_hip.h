@@ -163,6 +163,15 @@ int lbm_links_apply_bouzidi(void* d_f, const lbm_layout* layout, const uint32_t*
                             void* stream);
 
 /*
+ * The coefficients lbm_links_apply_bouzidi takes, from pylbm's per-link `dist`
+ * array of the Bouzidi methods (pylbm/boundary.py:525-535, the `s` it passes as
+ * `dist`, FP64): ca = sign * dist, cb = 1 - dist, rounded once to the compute
+ * type of `storage` (sign +1: Bouzidi_bounce_back, -1: Bouzidi_anti_bounce_back).
+ */
+int lbm_links_bouzidi_coefficients(const double* d_dist, double sign, void* d_ca, void* d_cb, int64_t n,
+                                   int storage, void* stream);
+
+/*
  * FP64 compensated (Neumaier) sums of every population over the interior.
  * d_partial needs 2*ns*(nblocks+1) doubles (nblocks from lbm_reduce_blocks),
  * h_sum/h_comp receive ns values each; the conserved sums are

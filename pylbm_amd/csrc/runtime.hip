@@ -239,6 +239,18 @@ static int bouzidi_launch(void* d_f, const uint32_t* st, const uint32_t* l0, con
   return 0;
 }
 
+// ca = sign * s and cb = 1 - s from the FP64 per-link s, each rounded once to C
+template <typename C>
+__global__ void __launch_bounds__(256) bouzidi_coefficients_kernel(const double* __restrict__ dist, double sign,
+                                                                   C* __restrict__ ca, C* __restrict__ cb,
+                                                                   int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double s = dist[t];
+  ca[t] = (C)(sign * s);
+  cb[t] = (C)(1.0 - s);
+}
+
 // Several independent link lists (no list reads or writes a slot another one writes)
 // in one launch: thread t belongs to the list j with end[j - 1] <= t < end[j] and
 // applies that list's link t - end[j - 1] exactly as links_kernel does (or, for a
@@ -610,6 +622,21 @@ int lbm_links_apply_bouzidi(void* d_f, const lbm_layout* layout, const uint32_t*
   hipStream_t s = (hipStream_t)stream;
   LBM_DISPATCH_STORAGE(layout->storage, bouzidi_launch, d_f, d_store, d_load0, d_load1, d_ca, d_cb, d_rhs,
                        n, two_phase, d_work, s);
+}
+
+int lbm_links_bouzidi_coefficients(const double* d_dist, double sign, void* d_ca, void* d_cb, int64_t n,
+                                   int storage, void* stream) {
+  if (n <= 0) return 0;
+  LBM_REQUIRE(d_dist && d_ca && d_cb, "Bouzidi coefficient arrays are NULL");
+  LBM_REQUIRE(n < ((int64_t)1 << 31) * 256, "link list too long");
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (storage == LBM_F64)
+    bouzidi_coefficients_kernel<double><<<blocks, 256, 0, s>>>(d_dist, sign, (double*)d_ca, (double*)d_cb, n);
+  else
+    bouzidi_coefficients_kernel<float><<<blocks, 256, 0, s>>>(d_dist, sign, (float*)d_ca, (float*)d_cb, n);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
 }
 
 int lbm_links_apply_sets(void* d_f, const lbm_layout* layout, const lbm_link_set* sets, int nsets,

@@ -15,6 +15,10 @@ argument names, over the C ABI of include/pylbm_hip.h:
 * ``bounce_back``, ``anti_bounce_back``   (f, istore, iload0, rhs, ncond)
                                           (boundary.py:430-469, 646-684)
 * ``neumann``, ``neumannx/y/z``           (f, istore, iload0, ncond) (boundary.py:787-895)
+* ``Bouzidi_bounce_back``                 (f, fcopy, istore, iload0, iload1, rhs, dist, ncond)
+                                          (boundary.py:537-623)
+* ``Bouzidi_anti_bounce_back``            (f, istore, iload0, iload1, rhs, dist, ncond)
+                                          (boundary.py:707-760)
 
 Buffers are device pointers (integers); ``istore`` / ``iload0`` are flat uint32
 element indices of the population layout (``pylbm_amd.boundary._flat`` converts
@@ -22,7 +26,11 @@ pylbm's ``(k, ix, iy[, iz])`` rows), ``rhs`` is in the compute type (double for
 FP64 storage, float otherwise) or None.  A list that reads slots it writes must
 be given resolved (``pylbm_amd.boundary.host_lists``) together with a workspace,
 the keyword-only ``work`` of ncond values (keyword-only, so that the argument
-names pylbm reads are exactly its own).  Every routine is asynchronous on the module's stream
+names pylbm reads are exactly its own).  For the Bouzidi routines ``dist`` is
+the FP64 device array of pylbm's per-link ``s``, ``iload1`` the second loads,
+and a list that loads slots it stores is given with ``work`` (pylbm's ``fcopy``
+semantics: every value is gathered before any store; ``fcopy`` itself is not
+read).  The coefficient arrays made from ``dist`` are kept per (dist, ncond).  Every routine is asynchronous on the module's stream
 and raises RuntimeError with the library's message on failure.
 """
 import ctypes
@@ -97,3 +105,36 @@ class HipModule:
         self._links(f, istore, iload0, None, 1.0, ncond, work, "neumann")
 
     neumannx = neumanny = neumannz = neumann
+
+    # ------------------------------- interpolated routines (boundary.py:472-760)
+    def _coefficients(self, dist, ncond, sign):
+        key = (int(dist), int(ncond), sign)
+        cache = self.__dict__.setdefault("_coef", {})
+        if key not in cache:
+            cs = 8 if self.layout.storage == rtm.LBM_F64 else 4
+            ca = self.rt.malloc(max(1, ncond) * cs)
+            cb = self.rt.malloc(max(1, ncond) * cs)
+            self._ok(self.core.lbm_links_bouzidi_coefficients(self._p(dist), sign, self._p(ca), self._p(cb),
+                                                              int(ncond), self.layout.storage, self.stream),
+                     "Bouzidi coefficients")
+            cache[key] = (ca, cb)
+        return cache[key]
+
+    def _bouzidi(self, f, istore, iload0, iload1, rhs, dist, ncond, work, sign, what):
+        ca, cb = self._coefficients(dist, ncond, sign)
+        self._ok(self.core.lbm_links_apply_bouzidi(self._p(f), ctypes.byref(self.layout), self._p(istore),
+                                                   self._p(iload0), self._p(iload1), self._p(ca), self._p(cb),
+                                                   self._p(rhs), int(ncond), 1 if work is not None else 0,
+                                                   self._p(work), self.stream), what)
+
+    def Bouzidi_bounce_back(self, f, fcopy, istore, iload0, iload1, rhs, dist, ncond, *, work=None):
+        self._bouzidi(f, istore, iload0, iload1, rhs, dist, ncond, work, 1.0, "Bouzidi_bounce_back")
+
+    def Bouzidi_anti_bounce_back(self, f, istore, iload0, iload1, rhs, dist, ncond, *, work=None):
+        self._bouzidi(f, istore, iload0, iload1, rhs, dist, ncond, work, -1.0, "Bouzidi_anti_bounce_back")
+
+    def free(self):
+        """Release the coefficient arrays made for the Bouzidi routines."""
+        for ca, cb in self.__dict__.pop("_coef", {}).values():
+            self.rt.free(ca)
+            self.rt.free(cb)

@@ -89,6 +89,9 @@ def test_module_object_uses_pylbm_argument_names():
         "neumannx": ["f", "istore", "iload0", "ncond"],
         "neumanny": ["f", "istore", "iload0", "ncond"],
         "neumannz": ["f", "istore", "iload0", "ncond"],
+        # pylbm/boundary.py:537-566 (_get_args locals) and 606-623, 743-760 (loop symbols)
+        "Bouzidi_bounce_back": ["f", "fcopy", "istore", "iload0", "iload1", "rhs", "dist", "ncond"],
+        "Bouzidi_anti_bounce_back": ["f", "istore", "iload0", "iload1", "rhs", "dist", "ncond"],
     }
     for name, args in expect.items():
         assert inspect.getfullargspec(getattr(HipModule, name)).args == ["self"] + args, name

@@ -366,7 +366,10 @@ def _call_genfunction(func, args):
 
 @pytest.mark.parametrize("make", [lambda: cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0)),
                                   lambda: cases.lid_cavity_d2q9(24),
-                                  lambda: cases.d1q3_advection(16, walls=True)])
+                                  lambda: cases.d1q3_advection(16, walls=True),
+                                  lambda: cases.with_bouzidi(cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0))),
+                                  lambda: cases.poiseuille_d2q9(ny=5, y_low=1.3, y_high=2.4,
+                                                                method=bc.BouzidiBounceBack)])
 def test_hip_module_drives_pylbm_step_loop(hip, make):
     """The generator module object (pylbm_amd.module.HipModule) driven the way pylbm's
     Simulation.one_time_step drives its generated module: update, each link routine
@@ -381,6 +384,9 @@ def test_hip_module_drives_pylbm_step_loop(hip, make):
     b._initialize()
     mod = HipModule.from_simulation(b)
     work = {id(dl): b.rt.malloc(8 * max(1, dl.n)) for dl in b._dev_links}
+    # pylbm's per-link s of the Bouzidi methods, in the order of the device lists
+    dist = {id(dl): b.rt.upload(np.asarray(dl.method.s, dtype=np.float64)[dl.order])
+            for dl in b._dev_links if dl.bouzidi}
     t = 0.0
     for _ in range(12):
         F, Fn = b.container.F, b.container.Fnew
@@ -390,6 +396,8 @@ def test_hip_module_drives_pylbm_step_loop(hip, make):
             if isinstance(dl.method, str):
                 continue   # the ghost copy list of the restricted fill: update wraps every slot
             bargs = {"f": F.ptr, "istore": dl.d_store, "iload0": dl.d_load, "rhs": dl.d_rhs, "ncond": dl.n}
+            if dl.bouzidi:
+                bargs.update(fcopy=F.ptr, iload1=dl.d_load1, dist=dist[id(dl)])
             routine = getattr(mod, dl.method.kind.name)
             if dl.conflict:
                 # a list that reads its own stores runs resolved, gather before scatter
@@ -403,8 +411,9 @@ def test_hip_module_drives_pylbm_step_loop(hip, make):
         a.one_time_step()
     for k in range(a.ns):
         assert np.array_equal(a.F[k], b.F[k]), k
-    for p in work.values():
+    for p in list(work.values()) + list(dist.values()):
         b.rt.free(p)
+    mod.free()
 
 
 @pytest.mark.parametrize("dtype", ["float64", "float32", "float32-shifted"])
