@@ -12,9 +12,11 @@
   Bouzidi moves them to within 0.08 of the true walls (the remaining offset is
   the known relaxation-dependent slip of linear interpolation).  Taking the
   wrong branch, the wrong second load or the wrong s moves the roots away.
-  (Walls with q < 1/2 on both sides: with a wall farther than half a link from
-  its fluid node the profile is not an exact parabola for plain bounce-back
-  either, so those geometries pin nothing here.)
+  (Walls with q < 1/2 on both sides: with q > 1/2 the diagonal links of the
+  edge columns that cross the periodic x face keep the box's periodic flag,
+  pylbm/domain.py:603-610 replacing a distance only by a smaller one, so those
+  links get no boundary value and the profile is not a parabola for plain
+  bounce-back either.)
 """
 import numpy as np
 import pytest
