@@ -108,17 +108,19 @@ class HipModule:
 
     # ------------------------------- interpolated routines (boundary.py:472-760)
     def _coefficients(self, dist, ncond, sign):
+        # The buffers are reused across calls; the coefficients are recomputed on every
+        # call (one small launch, ordered before the link launch on the same stream), so
+        # new values behind the same `dist` address are never served stale.
         key = (int(dist), int(ncond), sign)
         cache = self.__dict__.setdefault("_coef", {})
         if key not in cache:
             cs = 8 if self.layout.storage == rtm.LBM_F64 else 4
-            ca = self.rt.malloc(max(1, ncond) * cs)
-            cb = self.rt.malloc(max(1, ncond) * cs)
-            self._ok(self.core.lbm_links_bouzidi_coefficients(self._p(dist), sign, self._p(ca), self._p(cb),
-                                                              int(ncond), self.layout.storage, self.stream),
-                     "Bouzidi coefficients")
-            cache[key] = (ca, cb)
-        return cache[key]
+            cache[key] = (self.rt.malloc(max(1, ncond) * cs), self.rt.malloc(max(1, ncond) * cs))
+        ca, cb = cache[key]
+        self._ok(self.core.lbm_links_bouzidi_coefficients(self._p(dist), sign, self._p(ca), self._p(cb),
+                                                          int(ncond), self.layout.storage, self.stream),
+                 "Bouzidi coefficients")
+        return ca, cb
 
     def _bouzidi(self, f, istore, iload0, iload1, rhs, dist, ncond, work, sign, what):
         ca, cb = self._coefficients(dist, ncond, sign)
