@@ -28,9 +28,15 @@ post-collision state is M^-1 EQ(m_c); mass and momentum are conserved to
 round-off on periodic boxes (math.fsum); bounce-back on resting walls
 conserves mass; a time-dependent value whose function ignores t gives the
 static value's run bit for bit, and one that switches at t0 gives the static
-runs' states before t0.  The C1-C5 trajectories themselves have no reference
-fixture, so this driver's end-to-end output is "parity unpinned" beyond
-those checks.
+runs' states before t0.  tests/test_oracle_pins.py adds closed forms for the
+source half-steps (a body force raises the total momentum by exactly
+dt g sum(rho) per step; a uniform logistic source follows two Euler half-steps),
+the valued bounce-back / anti-bounce-back right-hand sides (2 w_k c_k.j / c_s^2
+and the even second-order equilibrium) and plane Couette flow.  End to end, the
+driver reproduces the reference's own fixtures (tests/reference/*.h5 of pylbm,
+converted under tests/golden/h5) at the reference's tolerance
+(tests/test_reference_fixtures.py).  The C1-C5 trajectories themselves have no
+reference fixture; they rest on those checks.
 """
 import math
 

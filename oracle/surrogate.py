@@ -19,7 +19,8 @@ normalised state in units of the increment scale.
 Pinned by: central finite differences of the loss with respect to inputs,
 weights and biases, SiLU' against a finite difference, a direct loop
 convolution on a tiny image, and unshuffle/shuffle round trips
-(tests/test_oracle_surrogate.py).
+(tests/test_oracle_surrogate.py); Adam against torch.optim.Adam on the CPU and
+its first step -lr sign(g) (tests/test_oracle_pins.py).
 """
 import numpy as np
 

@@ -138,6 +138,8 @@ class Scheme:
                 raise NotImplementedError("equilibrium %d depends on %s (only conserved moments are "
                                           "supported by the hip generator)" % (i, bad))
         s = [_fixed_point_subs(r, pairs) for r in self.s]
+        # a rate given as 0.0 is a Float, which sympy does not compare equal to 0
+        s = [sp.Integer(0) if (r.is_number and float(r) == 0.0) else r for r in s]
         for i in range(nc):
             if s[i] != 0:
                 raise ValueError("conserved moment %d has a non-zero relaxation rate" % i)
