@@ -133,8 +133,10 @@ int lbm_ghost_fill_all(void* d_f, const lbm_layout* layout, const lbm_velocities
 /*
  * One boundary link list, f[store[i]] = sign * f[load[i]] + rhs[i], where
  * store/load are element indices into the population array (population offset
- * included) and rhs may be NULL (all zero).  For the 16-bit deviation formats
- * rhs must already include the rest-population shift.  Replaces the generated
+ * included) and rhs may be NULL (all zero).  rhs is in the compute type; for the
+ * deviation formats (FP32, FP16 and BF16 deviations) it must already include the
+ * rest-population shift (lbm_links_prepare_rhs makes it from pylbm's FP64 rhs).
+ * Replaces the generated
  * link loops bounce_back (pylbm/boundary.py:430-469), anti_bounce_back
  * (646-684) and neumann / neumannx / neumanny / neumannz (787-895).
  * two_phase != 0 gathers every load into d_work (n values of the compute type)
@@ -149,7 +151,9 @@ int lbm_links_apply(void* d_f, const lbm_layout* layout, const uint32_t* d_store
  * One interpolated (Bouzidi) link list,
  *   f[store[i]] = ca[i] * f[load0[i]] + cb[i] * f[load1[i]] + rhs[i],
  * with ca, cb and rhs in the compute type (double for FP64 storage, float
- * otherwise; rhs may be NULL).  Replaces the generated loops Bouzidi_bounce_back
+ * otherwise; rhs may be NULL; for the deviation formats rhs includes the
+ * rest-population shift of lbm_links_prepare_rhs, which must then be applied
+ * even when pylbm's rhs is zero).  Replaces the generated loops Bouzidi_bounce_back
  * (pylbm/boundary.py:506-623: ca = s, cb = 1 - s, loads from a copy of f taken
  * before the loop) and Bouzidi_anti_bounce_back (687-760: ca = -s, cb = 1 - s),
  * where for a wall at distance q the host sets s = 2q, load1 = (ksym, x + 2 v_k)
@@ -170,6 +174,27 @@ int lbm_links_apply_bouzidi(void* d_f, const lbm_layout* layout, const uint32_t*
  */
 int lbm_links_bouzidi_coefficients(const double* d_dist, double sign, void* d_ca, void* d_cb, int64_t n,
                                    int storage, void* stream);
+
+/*
+ * pylbm's FP64 right-hand side of one link list (BoundaryMethod.rhs,
+ * pylbm/boundary.py:204-205, set by set_rhs 421-427 / 637-643 / 525-535 and
+ * 700-705) in the form lbm_links_apply / lbm_links_apply_bouzidi take it:
+ * d_out[i] in the compute type of the layout's storage.  The link kernels of the
+ * deviation formats (LBM_F32D, LBM_F16D, LBM_BF16D) combine stored deviations
+ * f - w, so for those the rest-population term is added first:
+ *   plain list (d_load1 NULL):  rhs[i] + sign w(load0[i]) - w(store[i])
+ *   Bouzidi list:               rhs[i] + (sign s w(load0[i]) + (1 - s) w(load1[i]) - w(store[i])),
+ *                               s = d_dist[i]
+ * with w(j) = h_wshift[population of element index j] (the rest populations
+ * w_k rho0 of lbm_params.wshift), evaluated in FP64 and rounded once.  d_rhs may
+ * be NULL (all zero; the shift is still applied).  For LBM_F64 / LBM_F32 storage
+ * this is a plain conversion (h_wshift and the index arrays may be NULL).  No
+ * reference counterpart of its own: the adapter between pylbm's rhs and the
+ * deviation storage of this backend.
+ */
+int lbm_links_prepare_rhs(const lbm_layout* layout, const double* h_wshift, const uint32_t* d_store,
+                          const uint32_t* d_load0, const uint32_t* d_load1, const double* d_dist, double sign,
+                          const double* d_rhs, void* d_out, int64_t n, void* stream);
 
 /*
  * FP64 compensated (Neumaier) sums of every population over the interior.

@@ -251,6 +251,47 @@ __global__ void __launch_bounds__(256) bouzidi_coefficients_kernel(const double*
   cb[t] = (C)(1.0 - s);
 }
 
+// pylbm's FP64 rhs of one link list in the form the link kernels take it: the
+// compute type, and for the deviation formats (which combine stored deviations)
+// the rest-population term a0 w(load0) [+ a1 w(load1)] - w(store) added, in the
+// association the host lists use (contraction off, so every product and sum
+// rounds once as numpy's do).  Plain lists: a0 = sign; Bouzidi lists (load1 set):
+// a0 = sign dist, a1 = 1 - dist.
+struct RestTable {
+  double w[LBM_MAX_Q];
+};
+
+__device__ __forceinline__ int population_of(const lbm_layout& L, uint32_t idx) {
+  if (L.pop_stride > L.row_stride) return (int)((int64_t)idx / L.pop_stride);
+  return (int)(((int64_t)idx / L.pop_stride) % L.ns);
+}
+
+template <typename C>
+__global__ void __launch_bounds__(256) rhs_prepare_kernel(const lbm_layout L, const RestTable W, int shift,
+                                                          const uint32_t* __restrict__ store,
+                                                          const uint32_t* __restrict__ load0,
+                                                          const uint32_t* __restrict__ load1,
+                                                          const double* __restrict__ dist, double sign,
+                                                          const double* __restrict__ rhs, C* __restrict__ out,
+                                                          int64_t n) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double r = rhs ? rhs[t] : 0.0;
+  if (shift) {
+    const double ws = W.w[population_of(L, store[t])];
+    const double w0 = W.w[population_of(L, load0[t])];
+    if (load1) {
+      const double s = dist[t];
+      const double w1 = W.w[population_of(L, load1[t])];
+      r = r + (((sign * s) * w0 + (1.0 - s) * w1) - ws);
+    } else {
+      r = (r + sign * w0) - ws;
+    }
+  }
+  out[t] = (C)r;
+}
+
 // Several independent link lists (no list reads or writes a slot another one writes)
 // in one launch: thread t belongs to the list j with end[j - 1] <= t < end[j] and
 // applies that list's link t - end[j - 1] exactly as links_kernel does (or, for a
@@ -635,6 +676,34 @@ int lbm_links_bouzidi_coefficients(const double* d_dist, double sign, void* d_ca
     bouzidi_coefficients_kernel<double><<<blocks, 256, 0, s>>>(d_dist, sign, (double*)d_ca, (double*)d_cb, n);
   else
     bouzidi_coefficients_kernel<float><<<blocks, 256, 0, s>>>(d_dist, sign, (float*)d_ca, (float*)d_cb, n);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int lbm_links_prepare_rhs(const lbm_layout* layout, const double* h_wshift, const uint32_t* d_store,
+                          const uint32_t* d_load0, const uint32_t* d_load1, const double* d_dist, double sign,
+                          const double* d_rhs, void* d_out, int64_t n, void* stream) {
+  int rc = check_layout(layout);
+  if (rc) return rc;
+  if (n <= 0) return 0;
+  LBM_REQUIRE(n < ((int64_t)1 << 31) * 256, "link list too long");
+  LBM_REQUIRE(d_out != nullptr, "rhs output array is NULL");
+  const int shift = is_deviation_storage(layout->storage) ? 1 : 0;
+  RestTable W{};
+  if (shift) {
+    LBM_REQUIRE(h_wshift != nullptr, "deviation storage needs the rest populations");
+    LBM_REQUIRE(d_store && d_load0, "store/load index arrays are NULL");
+    LBM_REQUIRE(!d_load1 || d_dist, "a Bouzidi list needs its dist array");
+    for (int k = 0; k < layout->ns; ++k) W.w[k] = h_wshift[k];
+  }
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (layout->storage == LBM_F64)
+    rhs_prepare_kernel<double><<<blocks, 256, 0, s>>>(*layout, W, shift, d_store, d_load0, d_load1, d_dist, sign,
+                                                      d_rhs, (double*)d_out, n);
+  else
+    rhs_prepare_kernel<float><<<blocks, 256, 0, s>>>(*layout, W, shift, d_store, d_load0, d_load1, d_dist, sign,
+                                                     d_rhs, (float*)d_out, n);
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -22,16 +22,29 @@ argument names, over the C ABI of include/pylbm_hip.h:
 
 Buffers are device pointers (integers); ``istore`` / ``iload0`` are flat uint32
 element indices of the population layout (``pylbm_amd.boundary._flat`` converts
-pylbm's ``(k, ix, iy[, iz])`` rows), ``rhs`` is in the compute type (double for
-FP64 storage, float otherwise) or None.  A list that reads slots it writes must
-be given resolved (``pylbm_amd.boundary.host_lists``) together with a workspace,
-the keyword-only ``work`` of ncond values (keyword-only, so that the argument
-names pylbm reads are exactly its own).  For the Bouzidi routines ``dist`` is
-the FP64 device array of pylbm's per-link ``s``, ``iload1`` the second loads,
-and a list that loads slots it stores is given with ``work`` (pylbm's ``fcopy``
-semantics: every value is gathered before any store; ``fcopy`` itself is not
-read).  The coefficient arrays made from ``dist`` are kept per (dist, ncond).  Every routine is asynchronous on the module's stream
-and raises RuntimeError with the library's message on failure.
+pylbm's ``(k, ix, iy[, iz])`` rows).  ``rhs`` is pylbm's own right-hand side:
+FP64, one value per link (``BoundaryMethod.rhs``, pylbm/boundary.py:204-205), or
+None.  For FP64 storage it goes to the link kernel as it is; for every other
+format the module first makes the compute-type rhs the kernel takes
+(``lbm_links_prepare_rhs``, one small launch on the same stream): FP32 values,
+and for the deviation formats (``float32-shifted``, ``float16``, ``bfloat16``)
+the rest-population term ``sign w(load) - w(store)`` (Bouzidi:
+``sign s w(load0) + (1 - s) w(load1) - w(store)``) added, because their link
+kernels combine stored deviations ``f - w``; that term is applied even when
+``rhs`` is None (an anti-bounce-back list without a value still needs
+``-2 w_k``).  A list that reads slots it writes must be given resolved
+(``pylbm_amd.boundary.host_lists``) together with a workspace, the keyword-only
+``work`` of ncond values (keyword-only, so that the argument names pylbm reads
+are exactly its own).  For the Bouzidi routines ``dist`` is the FP64 device
+array of pylbm's per-link ``s``, ``iload1`` the second loads, and a list that
+loads slots it stores is given with ``work`` (pylbm's ``fcopy`` semantics: every
+value is gathered before any store; ``fcopy`` itself is not read).  The
+coefficient and rhs buffers the module makes are reused across calls (grown to
+the largest ncond) and their contents recomputed on every call, so new values
+behind the same addresses are never served stale; they are released by
+:meth:`HipModule.free` or when the module is collected.  Every routine is
+asynchronous on the module's stream and raises RuntimeError with the library's
+message on failure.
 """
 import ctypes
 
@@ -90,7 +103,34 @@ class HipModule:
                                               self.stream), "update")
 
     # ------------------------------------- boundary routines (boundary.py:430-895)
+    def _buffer(self, name, nvals):
+        """A module-owned device buffer of at least nvals compute-type values, grown on demand."""
+        cs = 8 if self.layout.storage == rtm.LBM_F64 else 4
+        bufs = self.__dict__.setdefault("_bufs", {})
+        ptr, cap = bufs.get(name, (None, 0))
+        if cap < nvals:
+            if ptr:
+                self.rt.free(ptr)
+            cap = max(1, nvals)
+            ptr = self.rt.malloc(cap * cs)
+            bufs[name] = (ptr, cap)
+        return ptr
+
+    def _rhs(self, rhs, istore, iload0, iload1, dist, sign, ncond):
+        """The compute-type rhs the link kernel takes, from pylbm's FP64 rhs (or None)."""
+        if self.layout.storage == rtm.LBM_F64:
+            return rhs
+        if rhs is None and self.layout.storage not in rtm.DEVIATION_FORMATS:
+            return None
+        out = self._buffer("rhs", ncond)
+        w = (ctypes.c_double * rtm.MAX_Q)(*self.params.wshift)
+        self._ok(self.core.lbm_links_prepare_rhs(ctypes.byref(self.layout), w, self._p(istore), self._p(iload0),
+                                                 self._p(iload1), self._p(dist), float(sign), self._p(rhs),
+                                                 self._p(out), int(ncond), self.stream), "rhs")
+        return out
+
     def _links(self, f, istore, iload0, rhs, sign, ncond, work, what):
+        rhs = self._rhs(rhs, istore, iload0, None, None, sign, ncond)
         self._ok(self.core.lbm_links_apply(self._p(f), ctypes.byref(self.layout), self._p(istore),
                                            self._p(iload0), self._p(rhs), None, float(sign), int(ncond),
                                            1 if work is not None else 0, self._p(work), self.stream), what)
@@ -108,15 +148,9 @@ class HipModule:
 
     # ------------------------------- interpolated routines (boundary.py:472-760)
     def _coefficients(self, dist, ncond, sign):
-        # The buffers are reused across calls; the coefficients are recomputed on every
-        # call (one small launch, ordered before the link launch on the same stream), so
-        # new values behind the same `dist` address are never served stale.
-        key = (int(dist), int(ncond), sign)
-        cache = self.__dict__.setdefault("_coef", {})
-        if key not in cache:
-            cs = 8 if self.layout.storage == rtm.LBM_F64 else 4
-            cache[key] = (self.rt.malloc(max(1, ncond) * cs), self.rt.malloc(max(1, ncond) * cs))
-        ca, cb = cache[key]
+        # recomputed on every call (one small launch, ordered before the link launch
+        # on the same stream) into buffers reused across calls
+        ca, cb = self._buffer("ca%+d" % sign, ncond), self._buffer("cb%+d" % sign, ncond)
         self._ok(self.core.lbm_links_bouzidi_coefficients(self._p(dist), sign, self._p(ca), self._p(cb),
                                                           int(ncond), self.layout.storage, self.stream),
                  "Bouzidi coefficients")
@@ -124,6 +158,7 @@ class HipModule:
 
     def _bouzidi(self, f, istore, iload0, iload1, rhs, dist, ncond, work, sign, what):
         ca, cb = self._coefficients(dist, ncond, sign)
+        rhs = self._rhs(rhs, istore, iload0, iload1, dist, sign, ncond)
         self._ok(self.core.lbm_links_apply_bouzidi(self._p(f), ctypes.byref(self.layout), self._p(istore),
                                                    self._p(iload0), self._p(iload1), self._p(ca), self._p(cb),
                                                    self._p(rhs), int(ncond), 1 if work is not None else 0,
@@ -136,7 +171,16 @@ class HipModule:
         self._bouzidi(f, istore, iload0, iload1, rhs, dist, ncond, work, -1.0, "Bouzidi_anti_bounce_back")
 
     def free(self):
-        """Release the coefficient arrays made for the Bouzidi routines."""
-        for ca, cb in self.__dict__.pop("_coef", {}).values():
-            self.rt.free(ca)
-            self.rt.free(cb)
+        """Release the coefficient and rhs buffers the module made (after the stream's
+        work that uses them)."""
+        bufs = self.__dict__.pop("_bufs", {})
+        if bufs:
+            self.rt.sync()
+        for ptr, _ in bufs.values():
+            self.rt.free(ptr)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -96,6 +96,8 @@ CORE_SIGNATURES = {
     "lbm_links_apply": ([_P, ctypes.POINTER(LbmLayout), _P, _P, _P, _P, _D, _I64, _I, _P, _P], _I),
     "lbm_links_apply_bouzidi": ([_P, ctypes.POINTER(LbmLayout), _P, _P, _P, _P, _P, _P, _I64, _I, _P, _P], _I),
     "lbm_links_bouzidi_coefficients": ([_P, _D, _P, _P, _I64, _I, _P], _I),
+    "lbm_links_prepare_rhs": ([ctypes.POINTER(LbmLayout), ctypes.POINTER(_D), _P, _P, _P, _P, _D, _P, _P, _I64, _P],
+                              _I),
     "lbm_links_apply_sets": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmLinkSet), _I, _P], _I),
     "lbm_reduce_blocks": ([ctypes.POINTER(LbmLayout), ctypes.POINTER(_I64)], _I),
     "lbm_reduce_populations": ([_P, ctypes.POINTER(LbmLayout), _P, _P, ctypes.POINTER(_D),

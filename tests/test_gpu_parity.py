@@ -370,16 +370,19 @@ def _call_genfunction(func, args):
                                   lambda: cases.with_bouzidi(cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0))),
                                   lambda: cases.poiseuille_d2q9(ny=5, y_low=1.3, y_high=2.4,
                                                                 method=bc.BouzidiBounceBack)])
-def test_hip_module_drives_pylbm_step_loop(hip, make):
+@pytest.mark.parametrize("dtype", ["float64", "float32", "float32-shifted", "float16"])
+def test_hip_module_drives_pylbm_step_loop(hip, make, dtype):
     """The generator module object (pylbm_amd.module.HipModule) driven the way pylbm's
     Simulation.one_time_step drives its generated module: update, each link routine
-    with BoundaryMethod's argument names, one_time_step with BaseAlgorithm's; the
-    interior equals Simulation.one_time_step's bit for bit."""
+    with BoundaryMethod's argument names and pylbm's own FP64 rhs (no rest-population
+    shift: the module adds it for the deviation formats), one_time_step with
+    BaseAlgorithm's; the interior equals Simulation.one_time_step's bit for bit."""
     import inspect
+    from pylbm_amd.boundary import host_lists
     from pylbm_amd.module import HipModule
     d = make()
-    a = Simulation(d)
-    b = Simulation(d, initialize=False)
+    a = Simulation(d, dtype=dtype)
+    b = Simulation(d, dtype=dtype, initialize=False)
     b.fold_links = False
     b._initialize()
     mod = HipModule.from_simulation(b)
@@ -387,6 +390,19 @@ def test_hip_module_drives_pylbm_step_loop(hip, make):
     # pylbm's per-link s of the Bouzidi methods, in the order of the device lists
     dist = {id(dl): b.rt.upload(np.asarray(dl.method.s, dtype=np.float64)[dl.order])
             for dl in b._dev_links if dl.bouzidi}
+    # pylbm's FP64 rhs (BoundaryMethod.rhs) in the order of the device lists
+    rhs64 = {}
+    for dl in b._dev_links:
+        if isinstance(dl.method, str) or getattr(dl.method.kind, "rhs_sign", None) is None and not dl.bouzidi:
+            continue
+        if dl.bouzidi:
+            r = np.asarray(dl.method.rhs, dtype=np.float64)[dl.order]
+        elif dl.conflict:
+            r = host_lists(dl.method, dl.layout, np.zeros(b.ns))[3]
+        else:
+            r = np.asarray(dl.method.rhs, dtype=np.float64)
+            r = r[dl.order] if dl.order is not None else r
+        rhs64[id(dl)] = b.rt.upload(r.astype(np.float64))
     t = 0.0
     for _ in range(12):
         F, Fn = b.container.F, b.container.Fnew
@@ -395,7 +411,8 @@ def test_hip_module_drives_pylbm_step_loop(hip, make):
         for dl in b._dev_links:
             if isinstance(dl.method, str):
                 continue   # the ghost copy list of the restricted fill: update wraps every slot
-            bargs = {"f": F.ptr, "istore": dl.d_store, "iload0": dl.d_load, "rhs": dl.d_rhs, "ncond": dl.n}
+            bargs = {"f": F.ptr, "istore": dl.d_store, "iload0": dl.d_load, "rhs": rhs64.get(id(dl)),
+                     "ncond": dl.n}
             if dl.bouzidi:
                 bargs.update(fcopy=F.ptr, iload1=dl.d_load1, dist=dist[id(dl)])
             routine = getattr(mod, dl.method.kind.name)
@@ -411,9 +428,9 @@ def test_hip_module_drives_pylbm_step_loop(hip, make):
         a.one_time_step()
     for k in range(a.ns):
         assert np.array_equal(a.F[k], b.F[k]), k
-    for p in list(work.values()) + list(dist.values()):
-        b.rt.free(p)
     mod.free()
+    for p in list(work.values()) + list(dist.values()) + list(rhs64.values()):
+        b.rt.free(p)
 
 
 @pytest.mark.parametrize("dtype", ["float64", "float32", "float32-shifted"])
