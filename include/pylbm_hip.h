@@ -207,6 +207,23 @@ int lbm_reduce_blocks(const lbm_layout* layout, int64_t* nblocks);
 int lbm_reduce_populations(const void* d_f, const lbm_layout* layout, const double* d_wshift,
                            double* d_partial, double* h_sum, double* h_comp, void* stream);
 
+/*
+ * lbm_reduce_populations without the d_partial workspace and without any
+ * allocation: the partial sums go to a fixed scratch array of the library on the
+ * current device (2 * LBM_MAX_Q * 1025 doubles, 512.5 KiB, held by every GPU the
+ * library runs on until the process ends; lbm_set_device looks up its address).
+ * Same kernels, same grid and same summation order, so the same h_sum / h_comp
+ * bit for bit.  Layouts whose reduction would need more scratch are refused with
+ * an error (none does within LBM_MAX_Q populations).  Calls that overlap on one
+ * GPU (several host threads or streams at once) share the scratch array and race;
+ * each call writes every partial it reads before reading it, so nothing needs
+ * clearing between calls.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under graph capture, tight monitoring
+ * loops, harnesses that forbid hipMalloc).
+ */
+int lbm_reduce_populations_noalloc(const void* d_f, const lbm_layout* layout, const double* d_wshift,
+                                   double* h_sum, double* h_comp, void* stream);
+
 /* ---------------------------------------------------- generated, per scheme */
 /*
  * Run-time parameters of the generated kernels.  omega[i] = s_i and

@@ -3,6 +3,8 @@
 // (periodic ghost fill, boundary link lists, FP64 compensated population sums).
 #include <stdarg.h>
 
+#include <mutex>
+
 #include "lbm_device.h"
 
 namespace lbm {
@@ -388,6 +390,7 @@ __device__ __forceinline__ Acc acc_merge(Acc a, Acc b) {
 }
 
 constexpr int kRedThreads = 256;
+constexpr int64_t kMaxRedBlocks = 1024;   // blocks per population of the first reduction pass
 
 template <typename S>
 __global__ void __launch_bounds__(kRedThreads) reduce_pop_kernel(const S* __restrict__ f,
@@ -465,6 +468,56 @@ static int reduce_launch_any(int storage, const void* d_f, const lbm_layout* L, 
   LBM_DISPATCH_STORAGE(storage, reduce_launch, d_f, L, wshift, partial, grid, s);
 }
 
+// ---------------------------------------- scratch of the allocation-free reduction
+// The per-block (sum, compensation) pairs of every population plus the final pairs,
+// for the largest grid lbm_reduce_blocks gives (kMaxRedBlocks blocks) and LBM_MAX_Q
+// populations: 2 * 32 * 1025 doubles = 512.5 KiB, held by every GPU the library runs
+// on until the process ends.  In host code `scratch` names a host-side stand-in;
+// the device address comes from hipGetSymbolAddress, per device.
+constexpr int64_t kScratchDoubles = 2 * (int64_t)LBM_MAX_Q * (kMaxRedBlocks + 1);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up once per device)
+static int scratch_for_current_device(double** out) {
+  int dev = 0;
+  LBM_CHECK_HIP(hipGetDevice(&dev));
+  LBM_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d is beyond the scratch table (%d devices)", dev,
+              kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    LBM_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// both reduction entry points: partials of nb blocks per population into `partial`
+// (2 ns (nb + 1) doubles), the final pairs at its tail, copied back to the host
+static int reduce_into(const void* d_f, const lbm_layout* layout, const double* d_wshift, double* partial,
+                       int64_t nb, double* h_sum, double* h_comp, hipStream_t s) {
+  dim3 grid((unsigned)nb, (unsigned)layout->ns);
+  int rc = reduce_launch_any(layout->storage, d_f, layout, d_wshift, partial, grid, s);
+  if (rc) return rc;
+  LBM_CHECK_HIP(hipGetLastError());
+  double* out = partial + 2 * (int64_t)layout->ns * nb;
+  reduce_final_kernel<<<dim3((unsigned)layout->ns), kRedThreads, 0, s>>>(partial, nb, out);
+  LBM_CHECK_HIP(hipGetLastError());
+  double tmp[2 * LBM_MAX_Q];
+  LBM_CHECK_HIP(hipMemcpyAsync(tmp, out, sizeof(double) * 2 * layout->ns, hipMemcpyDeviceToHost, s));
+  LBM_CHECK_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < layout->ns; ++k) {
+    h_sum[k] = tmp[2 * k];
+    h_comp[k] = tmp[2 * k + 1];
+  }
+  return 0;
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -482,7 +535,9 @@ int lbm_device_count(int* count) {
 
 int lbm_set_device(int device) {
   LBM_CHECK_HIP(hipSetDevice(device));
-  return 0;
+  // set-up of the allocation-free entry points on this device
+  double* p = nullptr;
+  return scratch_for_current_device(&p);
 }
 
 int lbm_device_name(int device, char* buf, int len) {
@@ -760,7 +815,7 @@ int lbm_reduce_blocks(const lbm_layout* layout, int64_t* nblocks) {
   int rc = check_layout(layout);
   if (rc) return rc;
   const int64_t rows = interior(*layout, 0) * interior(*layout, 1);
-  *nblocks = rows < 1024 ? rows : 1024;
+  *nblocks = rows < kMaxRedBlocks ? rows : kMaxRedBlocks;
   return 0;
 }
 
@@ -770,23 +825,22 @@ int lbm_reduce_populations(const void* d_f, const lbm_layout* layout, const doub
   int rc = lbm_reduce_blocks(layout, &nb);
   if (rc) return rc;
   LBM_REQUIRE(d_partial && h_sum && h_comp, "NULL output");
-  hipStream_t s = (hipStream_t)stream;
-  dim3 grid((unsigned)nb, (unsigned)layout->ns);
-  rc = reduce_launch_any(layout->storage, d_f, layout, d_wshift, d_partial, grid, s);
+  return reduce_into(d_f, layout, d_wshift, d_partial, nb, h_sum, h_comp, (hipStream_t)stream);
+}
+
+int lbm_reduce_populations_noalloc(const void* d_f, const lbm_layout* layout, const double* d_wshift,
+                                   double* h_sum, double* h_comp, void* stream) {
+  int64_t nb = 0;
+  int rc = lbm_reduce_blocks(layout, &nb);
   if (rc) return rc;
-  LBM_CHECK_HIP(hipGetLastError());
-  // the final (sum, comp) pairs go to the tail of the partial buffer
-  double* out = d_partial + 2 * (int64_t)layout->ns * nb;
-  reduce_final_kernel<<<dim3((unsigned)layout->ns), kRedThreads, 0, s>>>(d_partial, nb, out);
-  LBM_CHECK_HIP(hipGetLastError());
-  double tmp[2 * LBM_MAX_Q];
-  LBM_CHECK_HIP(hipMemcpyAsync(tmp, out, sizeof(double) * 2 * layout->ns, hipMemcpyDeviceToHost, s));
-  LBM_CHECK_HIP(hipStreamSynchronize(s));
-  for (int k = 0; k < layout->ns; ++k) {
-    h_sum[k] = tmp[2 * k];
-    h_comp[k] = tmp[2 * k + 1];
-  }
-  return 0;
+  LBM_REQUIRE(h_sum && h_comp, "NULL output");
+  const int64_t need = 2 * (int64_t)layout->ns * (nb + 1);
+  LBM_REQUIRE(need <= kScratchDoubles, "the reduction needs %lld doubles of scratch, the maximum is %lld",
+              (long long)need, (long long)kScratchDoubles);
+  double* part = nullptr;
+  rc = scratch_for_current_device(&part);
+  if (rc) return rc;
+  return reduce_into(d_f, layout, d_wshift, part, nb, h_sum, h_comp, (hipStream_t)stream);
 }
 
 }  // extern "C"

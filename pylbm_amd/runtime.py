@@ -102,6 +102,8 @@ CORE_SIGNATURES = {
     "lbm_reduce_blocks": ([ctypes.POINTER(LbmLayout), ctypes.POINTER(_I64)], _I),
     "lbm_reduce_populations": ([_P, ctypes.POINTER(LbmLayout), _P, _P, ctypes.POINTER(_D),
                                 ctypes.POINTER(_D), _P], _I),
+    "lbm_reduce_populations_noalloc": ([_P, ctypes.POINTER(LbmLayout), _P, ctypes.POINTER(_D),
+                                        ctypes.POINTER(_D), _P], _I),
 }
 
 _F = ctypes.c_float
