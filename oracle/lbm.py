@@ -203,6 +203,10 @@ class Simulation:
         for k in range(ns):
             fp[k] = f[(k,) + self._interior(self.v[k])[1:]]
         m = self._f2m(fp)
+        if self.scheme.rel_vel is not None:
+            self._relative_velocity_collision(m)
+            fnew[self._interior()] = self._m2f(np.einsum("ij...,j...->i...", self._tmu, m))
+            return
         self._sources(m)
         eq = self._eq_full(m)
         for i in range(ns):
@@ -211,6 +215,30 @@ class Simulation:
                 m[i] = (1 - s) * m[i] + s * eq[i]
         self._sources(m)
         fnew[self._interior()] = self._m2f(m)
+
+    def _relative_velocity_collision(self, m):
+        """The collision of a relative-velocity scheme (pylbm/algorithm/pull.py:38-59 with
+        base.py:255-263, 316-331, 404-428) on m = M f (all rows absolute on entry):
+        u from the conserved moments, the non-conserved rows moved to the frame of u
+        (rows of T(u) M f), [source dt/2], relaxation towards T(u) eq, the conserved
+        rows replaced by their relative values (T(u) M f)[:nc] of the pulled f,
+        [source dt/2]; leaves m in the relative frame and T(-u) in self._tmu for
+        f = M^-1 T(-u) m."""
+        sch = self.scheme
+        nc, ns = sch.nconsm, self.ns
+        u = sch.relative_velocity([m[i] for i in range(nc)])
+        tu = sch.shift(u)
+        self._tmu = sch.shift(u, inverse=True)
+        mrel = np.einsum("ij...,j...->i...", tu, m)
+        m[nc:] = mrel[nc:]
+        self._sources(m)
+        eq = np.einsum("ij...,j...->i...", tu, self._eq_full(m))
+        for i in range(ns):
+            s = sch.s[i]
+            if s != 0:
+                m[i] = (1 - s) * m[i] + s * eq[i]
+        m[:nc] = mrel[:nc]
+        self._sources(m)
 
     def _sources(self, m):
         if not self.scheme.sources:

@@ -113,8 +113,111 @@ def poiseuille(dx, generator="hip"):
     }, [rho, qx, qy]
 
 
+def _geier_d2q9(dx, la, rho_o, mu, zeta):
+    """The 'Geier' D2Q9 moments of the lid-driven cavity and Karman demos (used with the
+    relative velocity q / rho): polynomials, equilibrium, relaxation rates
+    (demo/2D/lid_driven_cavity.py:137-165, demo/2D/Karman_vortex_street.py:138-166)."""
+    X, Y = sp.symbols("X, Y")
+    RHO, QX, QY = sp.symbols("rho, qx, qy")
+    LA = sp.symbols("lambda", constants=True)
+    UX, UY = QX / RHO, QY / RHO
+    RHOU2 = RHO * (UX ** 2 + UY ** 2)
+    polynomials = [1, X, Y, X ** 2 + Y ** 2, X * Y ** 2, Y * X ** 2, X ** 2 * Y ** 2, X ** 2 - Y ** 2, X * Y]
+    equilibrium = [
+        RHO, QX, QY,
+        RHOU2 + 2 / 3 * RHO * LA ** 2,
+        QX * (LA ** 2 / 3 + UY ** 2),
+        QY * (LA ** 2 / 3 + UX ** 2),
+        RHO * (LA ** 2 / 3 + UX ** 2) * (LA ** 2 / 3 + UY ** 2),
+        RHO * (UX ** 2 - UY ** 2),
+        RHO * UX * UY,
+    ]
+    dummy = 3.0 / (la * rho_o * dx)
+    s_1 = 1 / (0.5 + dummy * (zeta - 2 * mu / 3))
+    s_2 = 1 / (0.5 + dummy * mu)
+    return polynomials, equilibrium, [0.0, 0.0, 0.0, s_1, s_1, s_1, s_1, s_2, s_2], (RHO, QX, QY, LA)
+
+
+def lid_driven_cavity(dx, generator="hip"):
+    """D2Q9 'Geier' scheme with the relative velocity q / rho, Bouzidi bounce-back walls
+    and a Bouzidi lid carrying (rho_o, rho_o u_lid, 0) (demo/2D/lid_driven_cavity.py:27-246)."""
+    la, rho_o, driven_velocity = 1.0, 1.0, 0.05
+    mu = 5.0e-6
+    zeta = 100 * mu
+    polynomials, equilibrium, s, (RHO, QX, QY, LA) = _geier_d2q9(dx, la, rho_o, mu, zeta)
+
+    def bc_up(f, m, x, y, rho_o, driven_velocity):
+        m[RHO] = rho_o
+        m[QX] = rho_o * driven_velocity
+        m[QY] = 0.0
+
+    return {
+        "parameters": {LA: la},
+        "box": {"x": [0.0, 1.0], "y": [0.0, 1.0], "label": [0, 0, 0, 1]},
+        "space_step": dx,
+        "scheme_velocity": LA,
+        "schemes": [{
+            "velocities": list(range(9)),
+            "polynomials": polynomials,
+            "relaxation_parameters": s,
+            "equilibrium": equilibrium,
+            "conserved_moments": [RHO, QX, QY],
+        }],
+        "init": {RHO: rho_o, QX: 0.0, QY: 0.0},
+        "boundary_conditions": {
+            0: {"method": {0: bc.BouzidiBounceBack}},
+            1: {"method": {0: bc.BouzidiBounceBack}, "value": (bc_up, (rho_o, driven_velocity))},
+        },
+        "generator": generator,
+        "relative_velocity": [QX / RHO, QY / RHO],
+    }, [RHO, QX, QY]
+
+
+def karman_vortex_street(dx, generator="hip"):
+    """D2Q9 'Geier' scheme with the relative velocity q / rho around a cylinder: Bouzidi
+    bounce-back inflow and walls carrying (rho_o, rho_o u_o, 0), NeumannX outflow,
+    Bouzidi bounce-back on the circle (demo/2D/Karman_vortex_street.py:23-233)."""
+    from .domain import Circle
+    la, rho_o, u_o = 1.0, 1.0, 0.05
+    radius = 1.0 / 32
+    mu = 5.0e-6
+    zeta = 10 * mu
+    xmin, xmax, ymin, ymax = 0.0, 2.0, 0.0, 1.0
+    polynomials, equilibrium, s, (RHO, QX, QY, LA) = _geier_d2q9(dx, la, rho_o, mu, zeta)
+
+    def bc_in(f, m, x, y, rho_o, u_o):
+        m[RHO] = rho_o
+        m[QX] = rho_o * u_o
+        m[QY] = 0.0
+
+    return {
+        "box": {"x": [xmin, xmax], "y": [ymin, ymax], "label": [0, 1, 0, 0]},
+        "elements": [Circle([0.3, 0.5 * (ymin + ymax) + 2 * dx], radius, label=2)],
+        "space_step": dx,
+        "scheme_velocity": la,
+        "schemes": [{
+            "velocities": list(range(9)),
+            "polynomials": polynomials,
+            "relaxation_parameters": s,
+            "equilibrium": equilibrium,
+            "conserved_moments": [RHO, QX, QY],
+        }],
+        "parameters": {LA: la},
+        "init": {RHO: rho_o, QX: rho_o * u_o, QY: 0.0},
+        "boundary_conditions": {
+            0: {"method": {0: bc.BouzidiBounceBack}, "value": (bc_in, (rho_o, u_o))},
+            1: {"method": {0: bc.NeumannX}},
+            2: {"method": {0: bc.BouzidiBounceBack}},
+        },
+        "generator": generator,
+        "relative_velocity": [QX / RHO, QY / RHO],
+    }, [RHO, QX, QY]
+
+
 # name of the fixture -> (dictionary builder, dx, Tf)
 DEMOS = {
     "test1D_advection_reaction": (advection_reaction, 1.0 / 64, 0.5),
     "test2D_poiseuille": (poiseuille, 1.0 / 64, 0.5),
+    "test2D_lid_driven_cavity": (lid_driven_cavity, 1.0 / 64, 0.5),
+    "test2D_karman_vortex_street": (karman_vortex_street, 1.0 / 64, 0.5),
 }

@@ -340,6 +340,56 @@ def _deviation_collision_lines(spec: KernelSpec):
     return out, names
 
 
+def _relative_collision_lines(spec: KernelSpec):
+    """Collision of a relative-velocity scheme on m = M f (pylbm/algorithm/pull.py:38-59
+    with base.py:255-263, 316-331, 366-368, 404-428), in full values:
+
+    u = rel_vel(m_c); mr = T(u) m (the rows of Mu f = T(u) M f); the non-conserved
+    rows take their relative values; [source dt/2]; relaxed rows
+    m_i <- (1 - s_i) m_i + s_i (T(u) eq(m_c))_i; the conserved rows take their
+    relative values mr_c (restore_conserved_moments); [source dt/2]; then
+    ma = T(-u) m, whose M^-1 gives the new populations.
+
+    Returns (lines, names) with names[i] the C name of ma_i."""
+    ns, nc = spec.ns, spec.nconsm
+    msyms = spec.msyms
+    out = []
+    for d, e in enumerate(spec.rel_vel):
+        out.append("const C ru%d = %s;" % (d, _ccode(e)))
+    Tu = [[sp.sympify(c) for c in r] for r in spec.Tu]
+    Tmu = [[sp.sympify(c) for c in r] for r in spec.Tmu]
+
+    def rows(mat, vec, which, prefix):
+        exprs = [sum((mat[i][j] * vec[j] for j in range(ns) if mat[i][j] != 0), sp.Integer(0)) for i in which]
+        repl, red = sp.cse(exprs, symbols=sp.numbered_symbols(prefix), optimizations="basic")
+        lines = ["const C %s = %s;" % (sym, _ccode(e)) for sym, e in repl]
+        return lines, red
+
+    # relative moments of the pulled populations
+    lines, red = rows(Tu, msyms, range(ns), "tr_")
+    out += lines
+    out += ["const C mr%d = %s;" % (i, _ccode(e)) for i, e in zip(range(ns), red)]
+    out += ["m%d = mr%d;" % (i, i) for i in range(nc, ns)]
+
+    def sources():
+        for idx, expr in spec.sources:
+            out.append("m%d = m%d + half_dt * (%s);" % (idx, idx, _ccode(sp.sympify(expr))))
+
+    sources()
+    relaxed = [i for i in range(nc, ns) if spec.s[i] != 0]
+    eqv = [msyms[j] if j < nc else sp.sympify(spec.eq[j]) for j in range(ns)]
+    lines, red = rows(Tu, eqv, relaxed, "te_")
+    out += lines
+    for i, e in zip(relaxed, red):
+        out.append("m%d = one_minus[%d] * m%d + omega[%d] * (%s);" % (i, i, i, i, _ccode(e)))
+    out += ["m%d = mr%d;" % (i, i) for i in range(nc)]
+    sources()
+    lines, red = rows(Tmu, msyms, range(ns), "ta_")
+    out += lines
+    out += ["const C ma%d = %s;" % (i, _ccode(e)) for i, e in zip(range(ns), red)]
+    return out, ["ma%d" % i for i in range(ns)]
+
+
 def _equilibrium_lines(spec: KernelSpec):
     ns, nc = spec.ns, spec.nconsm
     rows = list(range(nc, ns))
@@ -848,24 +898,35 @@ def generate_source(spec: KernelSpec) -> str:
         loads.append("  }")
     loads.append("}")
     gdecl = ["S %s;" % ", ".join("G%d[VEC]" % k for k in range(ns))]
-    decode = ["C f%d = DEV ? Codec<S>::decode_dev(L%d[j], a.wshift[%d]) : Codec<S>::decode(L%d[j], a.wshift[%d]);"
-              % (k, k, k, k, k) for k in range(ns)]
+    rel = spec.rel_vel is not None
+    if rel:
+        # relative-velocity schemes collide full values in every format
+        decode = ["C f%d = Codec<S>::decode(L%d[j], a.wshift[%d]);" % (k, k, k) for k in range(ns)]
+    else:
+        decode = ["C f%d = DEV ? Codec<S>::decode_dev(L%d[j], a.wshift[%d]) : Codec<S>::decode(L%d[j], a.wshift[%d]);"
+                  % (k, k, k, k, k) for k in range(ns)]
     vstores = ["vstore<S, VEC>(fn + (a.dst_off[%d] + cell), G%d);" % (k, k) for k in range(ns)]
 
     b1 = _SumBuilder("p")
     f2m_lines = _transform(b1, M, fvars, mvars, ksym, sig_f2m)
     f2m_block = b1.lines + ["C %s = %s;" % (o, e) for o, e in f2m_lines]
-    coll = _collision_lines(spec, with_sources=True)
-    coll_dev, dm_names = _deviation_collision_lines(spec)
-    bd = _SumBuilder("rd")
-    m2f_dev_lines = _inverse_transform(bd, iM, dm_names, ["g%d" % k for k in range(ns)], ksym, sig_m2f,
-                                       base=fvars, tag="d")
-    m2f_dev_block = bd.lines + ["const C %s = %s;" % (o, e) for o, e in m2f_dev_lines]
+    if rel:
+        coll, ma_names = _relative_collision_lines(spec)
+    else:
+        coll, ma_names = _collision_lines(spec, with_sources=True), mvars
     b2 = _SumBuilder("r")
-    m2f_lines = _inverse_transform(b2, iM, mvars, ["g%d" % k for k in range(ns)], ksym, sig_m2f)
+    m2f_lines = _inverse_transform(b2, iM, ma_names, ["g%d" % k for k in range(ns)], ksym, sig_m2f)
     m2f_block = b2.lines + ["const C %s = %s;" % (o, e) for o, e in m2f_lines]
     stores = ["G%d[j] = Codec<S>::encode(g%d, a.wshift[%d]);" % (k, k, k) for k in range(ns)]
-    stores_dev = ["G%d[j] = Codec<S>::encode_dev(g%d, a.wshift[%d]);" % (k, k, k) for k in range(ns)]
+    if rel:
+        coll_dev, m2f_dev_block, stores_dev = coll, m2f_block, stores
+    else:
+        coll_dev, dm_names = _deviation_collision_lines(spec)
+        bd = _SumBuilder("rd")
+        m2f_dev_lines = _inverse_transform(bd, iM, dm_names, ["g%d" % k for k in range(ns)], ksym, sig_m2f,
+                                           base=fvars, tag="d")
+        m2f_dev_block = bd.lines + ["const C %s = %s;" % (o, e) for o, e in m2f_dev_lines]
+        stores_dev = ["G%d[j] = Codec<S>::encode_dev(g%d, a.wshift[%d]);" % (k, k, k) for k in range(ns)]
 
     # ---------------------------------------------------------- f2m / m2f / eq
     b3 = _SumBuilder("p")

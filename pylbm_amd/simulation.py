@@ -107,12 +107,19 @@ class Simulation:
         gen = str(dico.get("generator", "hip")).upper()
         if gen != "HIP":
             raise ValueError("pylbm_amd provides the 'hip' generator only (got %r)" % dico.get("generator"))
-        if sorder is not None and list(sorder) != list(range(len(sorder))):
-            raise NotImplementedError("the hip generator uses its own device layout; sorder must be None")
         if dtype not in STORAGE_BY_DTYPE:
             raise ValueError("dtype must be one of %s" % sorted(STORAGE_BY_DTYPE))
         self.rt = rtm.get(device)
         self.domain = Domain(dico, need_validation=False)
+        # pylbm's storage order (the axis order of its host arrays, pylbm/storage.py:
+        # 60-118) never changes a value or what the accessors return; the device
+        # layout here is the backend's own, so a valid order is accepted and kept
+        if sorder is not None:
+            if sorted(int(i) for i in sorder) != list(range(self.domain.dim + 1)):
+                raise ValueError("storage order must be a permutation of range(%d)" % (self.domain.dim + 1))
+            self.sorder = [int(i) for i in sorder]
+        else:
+            self.sorder = list(range(self.domain.dim + 1))
         Monitor.set_size(int(np.prod(self.domain.shape_in)))
         self.scheme = Scheme(dico, check_inverse=check_inverse, need_validation=False)
         if self.domain.dim != self.scheme.dim:
