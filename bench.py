@@ -110,7 +110,11 @@ def _cpu_baseline(target_s=8.0):
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    cores = min(cores, 16)
+    # every core of the process's CPU share; a thread limit set for the process
+    # (OMP_NUM_THREADS) is that share where the machine is shared between jobs
+    limit = os.environ.get("OMP_NUM_THREADS")
+    if limit and limit.isdigit() and int(limit) > 0:
+        cores = min(cores, int(limit))
     out = {}
     for threads in (1, cores):
         set_threads(threads)
@@ -127,6 +131,25 @@ def _cpu_baseline(target_s=8.0):
         sec = time.perf_counter() - t0
         out[threads] = (n ** 3 * steps / sec / 1e6, steps)
     return out, cores, n
+
+
+def _host_info():
+    """CPU model, logical CPUs of the machine and of this process."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def _link_timing(sim, reps=20):
@@ -202,6 +225,9 @@ def _configs_leg():
                 e["mass_rel_drift"] = float("%.3e" % ((sim.conserved_sums()[0] - m0) / m0))
             entry[dt] = e
             entry["cells"] = cells
+            if sim.domain.elements and "solid_fraction" not in entry:
+                inner = tuple(slice(h, -h) if h else slice(None) for h in sim.domain.stencil.vmax)
+                entry["solid_fraction"] = round(1.0 - float(np.mean(sim.domain.fluid[inner])), 5)
             sims[dt] = sim
         for dt in dtypes:
             if dt != "float64" and "float64" in sims:
@@ -413,7 +439,7 @@ def main():
     if not args.no_cpu_baseline:
         res, cores, cn = _cpu_baseline()
         cpu = {"value": round(res[cores][0], 2), "unit": "MLUPS", "cores": cores, "kind": "port",
-               "value_1core": round(res[1][0], 2),
+               "value_1core": round(res[1][0], 2), "host": _host_info(),
                "sample": "%d^3 D3Q19 MRT Taylor-Green, FP64 AoS, C restatement of pylbm's generated Cython "
                          "one_time_step + periodic ghost fill (oracle/cgen.py), %d steps on %d OpenMP threads, "
                          "%d steps on 1 thread" % (cn, res[cores][1], cores, res[1][1])}

@@ -228,16 +228,26 @@ def taylor_green_d3q19(n=256, u0=0.04, nu=0.005, generator="hip"):
 
 def sphere_centres(shape, rmin=4.0, rmax=12.0, solid_fraction=0.10, seed=2024):
     """Random sphere packing: centres uniform in the box, radii uniform in
-    [rmin, rmax], added (overlaps allowed) until the summed sphere volume
-    reaches solid_fraction of the box volume."""
+    [rmin, rmax], added (overlaps allowed) until the solid cells -- cells of the
+    box whose centre lies inside a sphere, the domain's own rule -- make up
+    solid_fraction of the box (dx = 1, cell i centred at i + 1/2)."""
     rng = np.random.default_rng(seed)
-    vol = float(np.prod(shape))
-    out, acc = [], 0.0
-    while acc < solid_fraction * vol:
+    solid = np.zeros(tuple(int(n) for n in shape), dtype=bool)
+    target = solid_fraction * solid.size
+    count = 0
+    out = []
+    while count < target:
         r = float(rng.uniform(rmin, rmax))
         c = [float(rng.uniform(0.0, s)) for s in shape]
         out.append((c, r))
-        acc += 4.0 / 3.0 * math.pi * r ** 3
+        lo = [max(0, int(math.floor(c[a] - r - 0.5))) for a in range(len(shape))]
+        hi = [min(int(shape[a]), int(math.ceil(c[a] + r + 0.5)) + 1) for a in range(len(shape))]
+        axes = np.meshgrid(*[np.arange(lo[a], hi[a]) + 0.5 - c[a] for a in range(len(shape))],
+                           sparse=True, indexing="ij")
+        inside = sum(x * x for x in axes) <= r * r
+        box = solid[tuple(slice(lo[a], hi[a]) for a in range(len(shape)))]
+        count += int(np.count_nonzero(inside & ~box))
+        box |= inside
     return out
 
 

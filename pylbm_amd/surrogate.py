@@ -5,7 +5,8 @@ moment fields (rho, ux, uy) at t to their increment over K lattice steps.  The
 fine state is unshuffled into coarse pixels of 64 channels (16 fine pixels x 4
 fields), so the convolutions run on a grid 16 times smaller and see 16 fine
 cells in every direction (farther than sound travels in K = 16 steps); the
-network is 64 -> C -> C -> C -> 64 channels with SiLU, C = 64 by default.
+network is 64 -> C -> C -> C -> 64 channels with SiLU, C = 32 or 64 (train_surrogate
+trains C = 32 by default; the network classes default to 64).
 Training and inference run on the project's own HIP kernels (FP16 or BF16
 MFMA operands, FP32 accumulation, FP32 master weights, Adam); training data
 are trajectories the HIP lattice Boltzmann run itself produces.
