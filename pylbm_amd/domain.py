@@ -80,6 +80,9 @@ class Domain:
         self._create_coords()
         self._build_mask()
         self._links = None
+        # pylbm's MPI topology (pylbm/domain.py:377-383), passed to H5File by pylbm's
+        # scripts; one GPU and one process here, so there is none
+        self.mpi_topo = None
 
     # ------------------------------------------------------------- coords
     def _create_coords(self):
