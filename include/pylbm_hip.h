@@ -256,6 +256,11 @@ typedef struct lbm_params {
   const uint32_t* bbtab_src;
   const void* bbtab_sign;
   const void* bbtab_rhs;
+  /* cell-centre coordinates for source terms that depend on space (pylbm's X, Y,
+   * Z): reference axis d of the cell at halo-inclusive index i is at
+   * origin[d] + dx * i (the coordinates of pylbm's Domain.coords_halo) */
+  double origin[3];
+  double dx;
 } lbm_params;
 
 /* information on the generated module (ns, nconsm, dim) */
@@ -267,7 +272,10 @@ int lbm_gen_info(int32_t* ns, int32_t* nconsm, int32_t* dim);
  * Replaces the generated one_time_step loop of
  * pylbm/algorithm/pull.py:11-59 (registered by pylbm/algorithm/base.py:551-628).
  * periodic_fold != 0 reads pulled values through the periodic wrap of every
- * axis instead of the halo (valid only when no link list exists).
+ * axis instead of the halo: valid for a periodic box, and for any box whose link
+ * lists are all folded into bbmask / bbtab (the ghost slots the pulls would read
+ * then hold the wrapped interior values, pylbm/storage.py:306-367, or are
+ * replaced by the folded link values), with the table sources in the interior.
  */
 int lbm_gen_step(const void* d_f, void* d_fnew, const lbm_layout* layout,
                  const lbm_params* params, int periodic_fold, void* stream);

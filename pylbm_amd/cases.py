@@ -226,6 +226,30 @@ def taylor_green_d3q19(n=256, u0=0.04, nu=0.005, generator="hip"):
     }
 
 
+def forced_periodic_d2q9(nx=12, ny=7, dx=0.5, g0=1e-3, seed=11, generator="hip"):
+    """Periodic D2Q9 MRT box driven by a body force that depends on space and time,
+    S(qx) = rho g0 (1 + Y) (1 + t), S(qy) = rho g0 X / 4 (X, Y cell centres, t the
+    time of the step), from a random density field."""
+    rng = np.random.default_rng(seed)
+    drho = rng.uniform(-0.05, 0.05, size=(nx + 2, ny + 2))
+
+    def rho(x, y):
+        return 1.0 + drho
+
+    t = sp.Symbol("t")
+    entry = d2q9_mrt(1.5, 1.4, 1.3, 1.7)
+    entry["source_terms"] = {QX: RHO * g0 * (1 + Y) * (1 + t), QY: RHO * g0 * X / 4}
+    return {
+        "parameters": {LA: 1},
+        "box": {"x": [0.0, nx * dx], "y": [0.0, ny * dx], "label": -1},
+        "space_step": dx,
+        "scheme_velocity": LA,
+        "schemes": [entry],
+        "init": {RHO: rho, QX: 0.0, QY: 0.0},
+        "generator": generator,
+    }
+
+
 def sphere_centres(shape, rmin=4.0, rmax=12.0, solid_fraction=0.10, seed=2024):
     """Random sphere packing: centres uniform in the box, radii uniform in
     [rmin, rmax], added (overlaps allowed) until the solid cells -- cells of the

@@ -23,7 +23,11 @@ CSRC = os.path.join(_PKG, "csrc")
 INCLUDE = os.path.join(os.path.dirname(_PKG), "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+# -ffp-contract=off: every product and sum of the generated arithmetic rounds on its
+# own, so the kernel variants that share it (halo / wrap reads, vector widths,
+# equilibrium + m2f fused or not) give the same bits wherever they compute the
+# same cell, whatever the compiler's scheduling of each variant
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 

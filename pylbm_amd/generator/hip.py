@@ -36,11 +36,12 @@ __all__ = ["generate_source", "spec_hash"]
 
 
 def _source_tag() -> str:
-    """Digest of the code generator and the device header it includes: a module
-    built from other sources never matches the cache key."""
+    """Digest of the code generator, its compile flags and the device header it
+    includes: a module built from other sources never matches the cache key."""
     h = hashlib.sha1()
     here = os.path.dirname(os.path.abspath(__file__))
-    for path in (os.path.abspath(__file__), os.path.join(here, "..", "csrc", "lbm_device.h"),
+    for path in (os.path.abspath(__file__), os.path.join(here, "build.py"),
+                 os.path.join(here, "..", "csrc", "lbm_device.h"),
                  os.path.join(here, "..", "..", "include", "pylbm_hip.h")):
         with open(path, "rb") as fh:
             h.update(fh.read())
@@ -70,6 +71,8 @@ class _Printer(C99CodePrinter):
         name = expr.name
         if name == "t":
             return "tt"
+        if name in ("X", "Y", "Z"):
+            return "xc%d" % "XYZ".index(name)
         if name.startswith("cr") and name[2:].isdigit():
             return "cref[%s]" % name[2:]
         return name
@@ -436,6 +439,7 @@ struct StepArgs {
   C omega[NS], one_minus[NS], wshift[NS];
   C cref[NCREF];
   C half_dt, t;
+  C xorg[3], dx;         // cell centre of reference axis d: xorg[d] + dx * (halo-inclusive index)
 };
 
 // Launch shape: LBM_BX threads along the contiguous axis x LBM_BY rows per
@@ -518,6 +522,7 @@ fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename 
 %(gdecl)s
 #pragma unroll
   for (int j = 0; j < VEC; ++j) {
+%(coords)s
 %(decode)s
 %(f2m)s
   if constexpr (DEV) {
@@ -630,6 +635,8 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   for (int c = 0; c < NCREF; ++c) a->cref[c] = p ? (C)p->cref[c] : (C)0;
   a->half_dt = p ? (C)(0.5 * p->dt) : (C)0;
   a->t = p ? (C)p->t : (C)0;
+  for (int d = 0; d < 3; ++d) a->xorg[d] = p ? (C)p->origin[d] : (C)0;
+  a->dx = p ? (C)p->dx : (C)0;
   a->bbmask = p ? p->bbmask : nullptr;
   a->bbtab_bits = p ? p->bbtab_bits : nullptr;
   a->bbtab_base = p ? p->bbtab_base : nullptr;
@@ -837,7 +844,7 @@ def generate_source(spec: KernelSpec) -> str:
     # cells that have table entries (ns <= 31)
     bbsub = []
     if ksym is not None:
-        bbsub = ["if constexpr (!FOLD) {",
+        bbsub = ["{",
                  "  if (a.bbmask) {",
                  "    const int64_t mc = ((int64_t)oo * a.nin[1] + mm) * a.nin[2] + i;",
                  "#pragma unroll",
@@ -952,12 +959,19 @@ def generate_source(spec: KernelSpec) -> str:
                    + ["C m%d;" % i for i in range(spec.nconsm, ns)])
     eqm2f_opaque = ["m%d = stored_moment(m%d);" % (i, i) for i in range(spec.nconsm, ns)]
 
+    # cell-centre coordinates of the reference axes the source terms use (pylbm's X, Y, Z)
+    used = set()
+    for _, e in spec.sources:
+        used |= {str(x) for x in sp.sympify(e).free_symbols}
+    canon_index = ["o", "m", "(ii + j)"]
+    coords = ["const C xc%d = a.xorg[%d] + a.dx * (C)%s;" % (d, d, canon_index[3 - dim + d])
+              for d in range(dim) if "XYZ"[d] in used]
     vel_txt = ", ".join("{%d, %d, %d}" % v for v in vel)
     return _TEMPLATE % dict(
         hash=spec_hash(spec), ns=ns, nc=spec.nconsm, dim=dim, paired=bool(sig_f2m),
         vel=vel_txt,
         loads=_indent(loads), bbsub=_indent(bbsub), bbsub_ok=int(ksym is not None),
-        gdecl=_indent(gdecl), decode=_indent(decode, 4), vstores=_indent(vstores),
+        gdecl=_indent(gdecl), coords=_indent(coords, 4), decode=_indent(decode, 4), vstores=_indent(vstores),
         f2m=_indent(f2m_block, 4), coll=_indent(coll, 4), coll_dev=_indent(coll_dev, 4),
         m2f=_indent(m2f_block, 4), m2f_dev=_indent(m2f_dev_block, 4), stores_dev=_indent(stores_dev, 4),
         stores=_indent(stores, 4),

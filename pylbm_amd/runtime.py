@@ -51,7 +51,8 @@ class LbmParams(ctypes.Structure):
                 ("wshift", ctypes.c_double * MAX_Q), ("cref", ctypes.c_double * MAX_Q),
                 ("dt", ctypes.c_double), ("t", ctypes.c_double), ("bbmask", ctypes.c_void_p),
                 ("bbtab_bits", ctypes.c_void_p), ("bbtab_base", ctypes.c_void_p), ("bbtab_src", ctypes.c_void_p),
-                ("bbtab_sign", ctypes.c_void_p), ("bbtab_rhs", ctypes.c_void_p)]
+                ("bbtab_sign", ctypes.c_void_p), ("bbtab_rhs", ctypes.c_void_p),
+                ("origin", ctypes.c_double * 3), ("dx", ctypes.c_double)]
 
 
 class LbmLinkSet(ctypes.Structure):

@@ -194,7 +194,13 @@ class Scheme:
             for lhs, rhs in sts.items():
                 idx = lhs if isinstance(lhs, int) else self.consm[sp.sympify(lhs) if isinstance(lhs, str) else lhs]
                 expr = rhs if isinstance(rhs, (int, float)) else _fixed_point_subs(sp.sympify(rhs), pairs + to_m)
-                sources.append((idx, sp.sympify(expr)))
+                expr = sp.sympify(expr)
+                allowed = set(msyms[:nc]) | set([X, Y, Z][: self.dim]) | {sp.Symbol("t")}
+                bad = expr.free_symbols - allowed
+                if bad:
+                    raise NotImplementedError("source term of moment %d depends on %s (conserved moments, "
+                                              "X/Y/Z and t are supported)" % (idx, bad))
+                sources.append((idx, expr))
         rel = Tus = Tmus = None
         if self.rel_vel is not None:
             rv = self.rel_vel if isinstance(self.rel_vel, (list, tuple)) else [self.rel_vel]

@@ -143,6 +143,10 @@ class Simulation:
             self.params.omega[i] = sv
             self.params.one_minus[i] = 1.0 - sv
         self.params.dt = self.dt
+        # coordinates of the cells (source terms that depend on X, Y, Z)
+        for d, c in enumerate(self.domain.coords_halo):
+            self.params.origin[d] = float(c[0])
+        self.params.dx = float(self.domain.dx)
         vmax = list(self.domain.stencil.vmax)
         # reference state of the FP32-compute formats: conserved moments
         # (rho0, 0, ...), its equilibrium populations w_k rho0
@@ -261,6 +265,16 @@ class Simulation:
         flags = merge_flags([(d.h_store, d.h_load, d.conflict) for d in active])
         for ls, flag in zip(self._link_sets, flags):
             ls.merge_prev = flag if self.merge_links else 0
+        # Wrap mode: when every boundary list is folded into the step, the step may read
+        # every pull through the periodic wrap of the interior instead of the halo.  A
+        # ghost slot the step would read holds the wrapped interior value (pylbm wraps
+        # the whole halo, pylbm/storage.py:306-367; the device fill and the ghost copy
+        # list write the same values), unless a link stores it -- and every link is
+        # folded, so those pulls take the folded value from the mask / table instead.
+        # So the step needs neither the ghost fill nor the ghost copy list: one launch
+        # per step, no halo line fetched, the same bits.
+        self._wrap = self.periodic_only or (bool(self.fold_links) and bool(self.bc.methods) and all(
+            d.fused or isinstance(d.method, str) for d in self._dev_links))
         for d in self._dev_links:
             d.h_store = d.h_load = d.h_sign = d.h_rhs = None
         self._need_init = False
@@ -372,11 +386,31 @@ class Simulation:
             cat = tuple(np.concatenate([t[q] for t in tab]) for q in range(5)) if tab else None
             self._upload_fold(mask, cat, owner)
 
+    def _wrap_ghost_sources(self, src):
+        """Table sources that are ghost slots, moved to the fully wrapped interior slot.
+
+        A source is a slot no link list stores (boundary.resolve_step), so a ghost
+        source holds the wrapped interior value at step time (the periodic-face fill
+        writes it there, as pylbm's wrap of the whole halo does): reading the
+        interior slot gives the same bits and lets the step skip the ghost fill."""
+        L, n, h, nin = self._fold_layout()
+        k, o, m, i = _unflat_fn(np.asarray(src, dtype=np.int64), L)
+        c = [np.asarray(o), np.asarray(m), np.asarray(i)]
+        ghost = np.zeros(c[0].shape, dtype=bool)
+        for a in range(3):
+            ghost |= (c[a] < h[a]) | (c[a] >= n[a] - h[a])
+            c[a] = np.where(c[a] < h[a], c[a] + nin[a], np.where(c[a] >= n[a] - h[a], c[a] - nin[a], c[a]))
+        if not ghost.any():
+            return src
+        wrapped = k * L.pop_stride + (c[0] * L.n[1] + c[1]) * L.row_stride + L.lead + c[2]
+        return np.where(ghost, wrapped, src)
+
     def _upload_fold(self, mask, tab, owner):
         """Device copies of the fold mask and table (cell, k, src, sign, rhs) -> params."""
         up = []
         if tab is not None and tab[0].size:
             cell, kk, src, sg, rh = tab
+            src = self._wrap_ghost_sources(src)
             o = np.lexsort((kk, cell))
             cell, kk, src, sg, rh = cell[o], kk[o], src[o], sg[o], rh[o]
             bits = np.zeros(mask.size, dtype=np.uint32)
@@ -471,6 +505,8 @@ class Simulation:
 
     # ---------------------------------------------------------- stepping
     def boundary_condition(self):
+        if self._wrap:
+            return   # every list folded into the step, which reads through the wrap
         f = self.container.F
         fn = self.rt.lib.lbm_ghost_fill if self._ghost_mode == 1 else self.rt.lib.lbm_ghost_fill_all
         self._check(fn(ctypes.c_void_p(f.ptr), ctypes.byref(f.layout), ctypes.byref(self.vel),
@@ -489,8 +525,8 @@ class Simulation:
         F, Fn = self.container.F, self.container.Fnew
         self.params.t = self.t
         self._check(self.module.lbm_gen_step(ctypes.c_void_p(F.ptr), ctypes.c_void_p(Fn.ptr),
-                                             ctypes.byref(F.layout), ctypes.byref(self.params), 0,
-                                             self.rt.stream), "one_time_step")
+                                             ctypes.byref(F.layout), ctypes.byref(self.params),
+                                             1 if self._wrap else 0, self.rt.stream), "one_time_step")
 
     def one_time_step(self):
         if self._need_init:
@@ -512,7 +548,12 @@ class Simulation:
         return not self.bc.methods and all(lab == -1 for lab in self.domain.box_label)
 
     def run(self, nsteps, fold=None):
-        """nsteps calls of one_time_step in one library call (same results)."""
+        """nsteps calls of one_time_step in one library call (same results).
+
+        fold=None reads the pulls through the periodic wrap whenever that gives the
+        same result (a periodic box, or every link list folded into the step);
+        fold=False keeps the halo path (ghost fill and link launches); fold=True
+        requires the wrap."""
         if self._need_init:
             self._initialize()
         if nsteps <= 0:
@@ -522,17 +563,18 @@ class Simulation:
                 self.one_time_step()
             return
         self._update_m = True
-        fold = self.periodic_only if fold is None else fold
-        if fold and not self.periodic_only:
-            raise ValueError("periodic fold requires a fully periodic box without link lists")
+        fold = self._wrap if fold is None else fold
+        if fold and not self._wrap:
+            raise ValueError("the periodic wrap needs a periodic box or every link list folded into the step")
         c = self.container
         self.params.t = self.t
         parity = ctypes.c_int(0)
         ghost = 0 if fold else self._ghost_mode
+        nmeth = 0 if fold else len(self._active_links)
         self._check(self.module.lbm_gen_run(ctypes.c_void_p(c.F.ptr), ctypes.c_void_p(c.Fnew.ptr),
                                             ctypes.byref(c.F.layout), ctypes.byref(self.vel),
                                             ctypes.byref(self.params), self._link_sets,
-                                            len(self._active_links), int(nsteps), ghost, 1 if fold else 0,
+                                            nmeth, int(nsteps), ghost, 1 if fold else 0,
                                             ctypes.byref(parity), self.rt.stream), "run")
         if parity.value:
             c.F, c.Fnew = c.Fnew, c.F

@@ -561,3 +561,55 @@ def test_relative_velocity_parity(hip, demo, dtype):
     _march(sim, ref, 40, use_run=True)
     errs = _compare(sim, ref, syms, REL_BARS[dtype])
     print(demo, dtype, errs)
+
+
+@pytest.mark.parametrize("table_max", [None, 0])
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("name", ["spheres", "karman", "d1q3_walls", "lid", "vectorial"])
+def test_wrap_mode_equals_halo_path(hip, monkeypatch, name, dtype, table_max):
+    """With every link list folded into the step, the step reads its pulls through the
+    periodic wrap of the interior (no ghost fill, no ghost copy list, no link launch):
+    the populations equal the halo path's (fold=False: ghost fill, link launches,
+    halo reads) bit for bit.  table_max=0 forces the per-list folding (the path of
+    the full-size sphere packing, whose ghost copy list is then dropped)."""
+    from pylbm_amd import simulation as simmod
+    if table_max is not None:
+        monkeypatch.setattr(simmod, "FOLD_TABLE_MAX", table_max)
+    d = {"spheres": lambda: cases.sphere_packing_d3q19(24, 16, 12, seed=5, solid_fraction=0.2),
+         "karman": lambda: cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0)),
+         "d1q3_walls": lambda: cases.d1q3_advection(16, walls=True),
+         "lid": lambda: cases.lid_cavity_d2q9(24),
+         "vectorial": lambda: cases.d2q5q9_vectorial(16, walls=True)}[name]()
+    a = Simulation(d, dtype=dtype)
+    b = Simulation(d, dtype=dtype)
+    if name in ("spheres", "lid"):
+        assert a._wrap
+    if a._wrap:
+        for _ in range(2):
+            a.one_time_step()
+            b.one_time_step()
+        a.run(25)
+        b.run(25, fold=False)
+        for k in range(a.ns):
+            assert np.array_equal(a.F[k], b.F[k]), k
+
+
+# Source terms that depend on the cell centres and on t (pylbm binds X, Y, Z, t in
+# source_terms, pylbm/algorithm/base.py:141-151, 439-495).  FP64 at the FP64 bar;
+# FP32 deviations: measured on MI355X, bar x ~3.
+SPACE_TIME_BARS = {"float64": TOL64, "float32-shifted": 1e-5}
+
+
+@pytest.mark.parametrize("dtype", sorted(SPACE_TIME_BARS))
+def test_space_time_source_parity(hip, dtype):
+    d = cases.forced_periodic_d2q9()
+    sim, ref = Simulation(d, dtype=dtype), Oracle(d)
+    _march(sim, ref, 30, use_run=True)
+    errs = _compare(sim, ref, [cases.RHO, cases.QX, cases.QY], SPACE_TIME_BARS[dtype])
+    print("space-time source", dtype, errs)
+    # one_time_step takes the same path
+    a = Simulation(d, dtype=dtype)
+    for _ in range(30):
+        a.one_time_step()
+    for k in range(a.ns):
+        assert np.array_equal(a.F[k], sim.F[k]), k

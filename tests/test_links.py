@@ -402,3 +402,71 @@ def test_bouzidi_anti_bounce_back_refuses_earlier_store_loads():
     L = make_layout(int(dom.stencil.nv_ptr[-1]), tuple(dom.shape_halo), list(dom.stencil.vmax), LBM_F64)
     with pytest.raises(NotImplementedError):
         bouzidi_lists(pm[0], L, np.zeros(9))
+
+
+@pytest.mark.parametrize("name", ["spheres", "lid", "d1q3_walls", "karman", "vectorial", "neumann_periodic"])
+def test_wrap_mode_pulls_equal_halo_pulls(name):
+    """The step's pulled values with every list folded and the pulls read through the
+    periodic wrap (the wrap mode of Simulation: table sources in ghost slots moved to
+    the wrapped interior slot) equal, for every interior cell and population, the
+    pulls of pylbm's sequence: wrap of the whole halo (the oracle's ghost fill,
+    pylbm/storage.py:306-367), every list in order, then f[k, x - v_k]."""
+    from oracle.lbm import Simulation as O
+    from pylbm_amd.boundary import _unflat, host_lists, resolve_step, unwritten_ghost_reads
+    d = {"spheres": lambda: cases.sphere_packing_d3q19(20, 14, 12, seed=3, solid_fraction=0.25),
+         "lid": lambda: cases.lid_cavity_d2q9(12),
+         "d1q3_walls": lambda: cases.d1q3_advection(10, walls=True),
+         "karman": lambda: cases.karman_d2q9(40, 20, radius=4.0, center=(10.0, 10.0)),
+         "vectorial": lambda: cases.d2q5q9_vectorial(12, walls=True),
+         "neumann_periodic": _neumann_periodic}[name]()
+    o = O(d)
+    dom, pm = _product(d)
+    for m in pm:
+        m.fix_iload()
+    lay = _HostLayout(o.ns, o.F.shape[1:])
+    lists = []
+    st, ld = unwritten_ghost_reads(dom, dom.stencil, pm, lay)
+    if st.size:
+        lists.append(host_lists("ghost_copy", lay, None, flat=(st, ld)))
+    rng = np.random.default_rng(4)
+    for m in pm:
+        m.rhs = rng.normal(size=m.rhs.shape)
+        lists.append(host_lists(m, lay, None))
+    F0 = rng.normal(size=o.F.shape)
+    # pylbm: wrap every halo slot, the lists in order, then the pulls
+    seq = F0.copy()
+    o.ghost_fill(seq)
+    for lst in lists:
+        _apply(seq, lst)
+    # wrap mode: folded values where a list stores the pulled slot, else the wrapped read
+    slot, org, sg, rh, has, bad = resolve_step([(x[0], x[1], x[2], x[3] if np.any(x[3] != 0) else None)
+                                                for x in lists])
+    n = np.array(lay.n)
+    h = np.array([0] * (3 - o.dim) + [int(v) for v in o.h])
+    nin = n - 2 * h
+
+    def wrap(c):
+        return [np.where(c[a] < h[a], c[a] + nin[a], np.where(c[a] >= n[a] - h[a], c[a] - nin[a], c[a]))
+                for a in range(3)]
+
+    k, oo, mm, ii = _unflat(org, lay)
+    w = wrap([np.asarray(oo), np.asarray(mm), np.asarray(ii)])
+    org_w = k * lay.pop_stride + (w[0] * n[1] + w[1]) * lay.row_stride + w[2]
+    folded = dict(zip(slot.tolist(), (sg * F0.reshape(-1)[org_w] + np.where(has, rh, 0.0)).tolist()))
+    vel = np.asarray(o.v)
+    v3 = np.zeros((o.ns, 3), dtype=np.int64)
+    v3[:, 3 - o.dim:] = vel
+    grid = np.stack(np.meshgrid(*[np.arange(h[a], n[a] - h[a]) for a in range(3)], indexing="ij"), axis=0)
+    grid = grid.reshape(3, -1)
+    flat_seq = seq.reshape(-1)
+    for q in range(o.ns):
+        src = grid - v3[q][:, None]
+        idx = q * lay.pop_stride + (src[0] * n[1] + src[1]) * lay.row_stride + src[2]
+        want = flat_seq[idx]
+        ws = wrap(src)
+        idx_w = q * lay.pop_stride + (ws[0] * n[1] + ws[1]) * lay.row_stride + ws[2]
+        got = F0.reshape(-1)[idx_w].copy()
+        for j, s in enumerate(idx.tolist()):
+            if s in folded:
+                got[j] = folded[s]
+        assert np.array_equal(got, want), (name, q, int(np.sum(got != want)))

@@ -108,6 +108,31 @@ def test_logistic_source_two_euler_half_steps():
     assert abs(u - u0) > 1e-2
 
 
+def test_space_time_source_momentum_growth():
+    """A force depending on the cell centres and on t, S(qx) = rho g0 (1 + Y)(1 + t),
+    S(qy) = rho g0 X / 4, on a periodic box: streaming permutes the populations and
+    relaxation keeps the conserved moments, so over the step from t_n the total
+    momentum grows by exactly dt sum_x rho^{n+1}(x) S(x, t_n) / rho -- the two half
+    steps at the same time t_n and the density after streaming (the new state's).
+    Swapping X and Y, taking t_{n+1}, or one half step fails it."""
+    g0 = 1e-3
+    d = cases.forced_periodic_d2q9(g0=g0, generator="numpy")
+    sim = Simulation(d)
+    X, Y = np.meshgrid(*sim.domain.coords, indexing="ij")
+    for _ in range(5):
+        t_n = sim.t
+        before = sim.conserved_sums()
+        sim.one_time_step()
+        after = sim.conserved_sums()
+        rho = sim.m(cases.RHO)
+        want_x = sim.dt * math.fsum((rho * g0 * (1 + Y) * (1 + t_n)).ravel())
+        want_y = sim.dt * math.fsum((rho * g0 * X / 4).ravel())
+        assert abs((after[1] - before[1]) - want_x) <= 1e-12 * abs(want_x), (after[1] - before[1], want_x)
+        assert abs((after[2] - before[2]) - want_y) <= 1e-12 * abs(want_y), (after[2] - before[2], want_y)
+        assert abs(after[0] - before[0]) <= 1e-13 * abs(before[0])
+    assert sim.t > 0
+
+
 # -------------------------------------------------------- valued boundaries
 
 def _lid_d2q9(u_lid, n=8, s_nu=1.2):
