@@ -256,11 +256,20 @@ typedef struct lbm_params {
   const uint32_t* bbtab_src;
   const void* bbtab_sign;
   const void* bbtab_rhs;
+  /* NULL, or per table entry its source relative to the pulling cell, for the
+   * temporally blocked run: bits 0-7 the source population, bits 8-15 and 16-23
+   * the mid / inner offsets + 128 (within one cell; 2D schemes) */
+  const uint32_t* bbtab_rel;
   /* cell-centre coordinates for source terms that depend on space (pylbm's X, Y,
    * Z): reference axis d of the cell at halo-inclusive index i is at
    * origin[d] + dx * i (the coordinates of pylbm's Domain.coords_halo) */
   double origin[3];
   double dx;
+  /* workgroup order of the step: 1 = each of the 8 XCDs takes a contiguous range
+   * of rows (their shared neighbour rows then meet in one L2), 0 = the hardware's
+   * round-robin order, -1 = the default (1 on the halo path, 0 on the wrap path) */
+  int32_t xcd_remap;
+  int32_t _pad_params;
 } lbm_params;
 
 /* information on the generated module (ns, nconsm, dim) */
@@ -330,6 +339,20 @@ typedef struct lbm_link_set {
  */
 int lbm_links_apply_sets(void* d_f, const lbm_layout* layout, const lbm_link_set* sets, int nsets,
                          void* stream);
+
+/*
+ * nsteps time steps of a 2D scheme on the wrap path (a periodic box, or every link
+ * list folded into bbmask / bbtab with bbtab_rel set), steps_per_launch (<= 16) of
+ * them per launch: each workgroup loads a tile_m x tile_i tile of the interior with
+ * a halo of steps_per_launch cells into LDS and advances it there, with the
+ * per-cell arithmetic of lbm_gen_step (the same bits); the remaining
+ * nsteps % steps_per_launch steps run as lbm_gen_step launches.  This is
+ * pylbm/simulation.py:392-420 repeated, for grids whose step is too small to fill
+ * the GPU.  *parity as for lbm_gen_run; params->t advances by nsteps dt.
+ */
+int lbm_gen_run_blocked(void* d_f, void* d_fnew, const lbm_layout* layout, lbm_params* params,
+                        int64_t nsteps, int steps_per_launch, int tile_m, int tile_i, int* parity,
+                        void* stream);
 
 int lbm_gen_run(void* d_f, void* d_fnew, const lbm_layout* layout, const lbm_velocities* vel,
                 lbm_params* params, const lbm_link_set* links, int nmeth, int64_t nsteps,

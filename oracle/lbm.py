@@ -112,7 +112,7 @@ class Simulation:
         m = np.zeros_like(self.F)
         mv = _Moments(m, self.scheme.consm)
         init = self.dico.get("init", {})
-        for k, v in init.items():
+        for k, v in (init.items() if self.dico.get("inittype", "moments") == "moments" else ()):
             if isinstance(v, tuple):
                 mv[k] = v[0](*(tuple(coords) + (tuple(v[1]) if len(v) == 2 else ())))
             elif callable(v):
@@ -123,7 +123,15 @@ class Simulation:
             m = self._eq_full(m)
             self.F[...] = self._m2f(m)
         else:
-            raise NotImplementedError("oracle: distributions init")
+            # the populations themselves, keyed by their index (simulation.py:302-312)
+            for k, v in init.items():
+                if isinstance(v, tuple):
+                    val = v[0](*(tuple(coords) + (tuple(v[1]) if len(v) == 2 else ())))
+                elif callable(v):
+                    val = v(*coords)
+                else:
+                    val = v
+                self.F[int(k)] = val
         self.Fnew[...] = self.F
 
     def _prepare_rhs(self, meth, t=None):

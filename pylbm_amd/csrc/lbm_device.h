@@ -248,6 +248,17 @@ __host__ __device__ __forceinline__ int64_t cell_offset(const lbm_layout& L, int
   return (o * L.n[1] + m) * L.row_stride + L.lead + i;
 }
 
+// one past the largest element index of the array (any arrangement)
+__host__ __device__ __forceinline__ int64_t layout_elements(const lbm_layout& L) {
+  const int64_t rows = L.n[0] * L.n[1];
+  return (L.ns - 1) * L.pop_stride + (rows - 1) * L.row_stride + L.pitch;
+}
+
+// true when every element index and every offset between two elements fits int32
+__host__ __device__ __forceinline__ bool layout_fits_int32(const lbm_layout& L) {
+  return layout_elements(L) < ((int64_t)1 << 31);
+}
+
 inline int check_layout(const lbm_layout* L) {
   LBM_REQUIRE(L != nullptr, "layout is NULL");
   LBM_REQUIRE(L->ns > 0 && L->ns <= LBM_MAX_Q, "ns=%d out of range [1, %d]", L->ns, LBM_MAX_Q);
@@ -263,10 +274,10 @@ inline int check_layout(const lbm_layout* L) {
   const bool planes = L->row_stride >= L->pitch && L->pop_stride >= rows * L->row_stride;
   const bool interleaved = L->pop_stride >= L->pitch && L->row_stride >= L->ns * L->pop_stride;
   LBM_REQUIRE(planes || interleaved, "pop_stride / row_stride overlap");
-  // every element index must fit the 32-bit offsets used by the kernels
-  const int64_t last = (L->ns - 1) * L->pop_stride + (rows - 1) * L->row_stride + L->pitch;
-  LBM_REQUIRE(last < ((int64_t)1 << 31), "array of %lld elements exceeds the 2^31 element index range",
-              (long long)last);
+  // element indices are 64-bit: arrays of 2^31 elements and more take the kernels'
+  // 64-bit index variants (layout_elements), link lists their own 32-bit check
+  const int64_t last = layout_elements(*L);
+  LBM_REQUIRE(last > 0 && last < ((int64_t)1 << 62), "array of %lld elements out of range", (long long)last);
   return 0;
 }
 

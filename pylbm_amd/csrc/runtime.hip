@@ -67,29 +67,29 @@ __global__ void __launch_bounds__(256) ghost_fill_kernel(const T* __restrict__ s
                                                          T* __restrict__ dst_f, lbm_layout L,
                                                          lbm_velocities V, ShellDesc S,
                                                          int all_pops) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int)(S.nA + S.nB + S.nC)) return;
-  const int n0 = (int)L.n[0], n1 = (int)L.n[1], n2 = (int)L.n[2];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S.nA + S.nB + S.nC) return;
+  const int64_t n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
   const int h0 = L.halo[0], h1 = L.halo[1], h2 = L.halo[2];
-  int o, m, i;
-  if (t < (int)S.nA) {  // outer ghost slabs: layers x n1 x n2
-    int r = t;
+  int64_t o, m, i;
+  if (t < S.nA) {  // outer ghost slabs: layers x n1 x n2
+    int64_t r = t;
     i = r % n2;
     r /= n2;
     m = r % n1;
     r /= n1;
-    o = slab_coord(r, n0, h0, S.sidesA);
-  } else if (t < (int)(S.nA + S.nB)) {  // mid ghost slabs: (n0-2h0) x layers x n2
+    o = slab_coord((int)r, (int)n0, h0, S.sidesA);
+  } else if (t < S.nA + S.nB) {  // mid ghost slabs: (n0-2h0) x layers x n2
     const int nl = h1 * nsides(S.sidesB);
-    int r = t - (int)S.nA;
+    int64_t r = t - S.nA;
     i = r % n2;
     r /= n2;
-    m = slab_coord(r % nl, n1, h1, S.sidesB);
+    m = slab_coord((int)(r % nl), (int)n1, h1, S.sidesB);
     o = h0 + r / nl;
   } else {  // inner ghost slabs: (n0-2h0) x (n1-2h1) x layers
     const int nl = h2 * nsides(S.sidesC);
-    int r = t - (int)(S.nA + S.nB);
-    i = slab_coord(r % nl, n2, h2, S.sidesC);
+    int64_t r = t - (S.nA + S.nB);
+    i = slab_coord((int)(r % nl), (int)n2, h2, S.sidesC);
     r /= nl;
     m = h1 + r % (n1 - 2 * h1);
     o = h0 + r / (n1 - 2 * h1);
@@ -99,9 +99,9 @@ __global__ void __launch_bounds__(256) ghost_fill_kernel(const T* __restrict__ s
   ghost_axis(o, n0, h0, &wo, &d0);
   ghost_axis(m, n1, h1, &wm, &d1);
   ghost_axis(i, n2, h2, &wi, &d2);
-  const int dst = (int)cell_offset(L, o, m, i);
-  const int src = (int)cell_offset(L, wo, wm, wi);
-  const int ps = (int)L.pop_stride;
+  const int64_t dst = cell_offset(L, o, m, i);
+  const int64_t src = cell_offset(L, wo, wm, wi);
+  const int64_t ps = L.pop_stride;
   T val[QMAX];
   bool take[QMAX];
 #pragma unroll

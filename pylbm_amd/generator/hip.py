@@ -97,6 +97,8 @@ class _Printer(C99CodePrinter):
             return "(%s)" % prod if n > 0 else "(C(1.0)/(%s))" % prod
         if e == sp.Rational(1, 2):
             return "sqrt(%s)" % self._print(b)
+        if e.is_number and e.is_real:
+            return "pow(%s, %s)" % (self._print(b), self._print(e))
         raise NotImplementedError("power %s not supported" % expr)
 
 
@@ -420,22 +422,26 @@ using namespace lbm;
 #define NCONSM %(nc)d
 #define NCREF (NCONSM > 0 ? NCONSM : 1)
 #define DIM %(dim)d
+#define TB_UNIT_VELOCITIES %(unit_vel)d  // every velocity component in {-1, 0, 1}
+#define TB_THREADS 512                 // threads of a temporally blocked workgroup
+#define TB_MAX_NT 16                   // steps per temporally blocked launch, at most
 
 namespace {
 
 template <typename C>
 struct StepArgs {
-  int32_t src_off[NS];   // pull source offset of population k relative to the cell
-  int32_t dst_off[NS];   // k * pop_stride
+  int64_t src_off[NS];   // pull source offset of population k relative to the cell
+  int64_t dst_off[NS];   // k * pop_stride
   int32_t nin[3];        // interior extents
   int32_t halo[3];
-  int32_t n1, rs, lead, _pad;      // rs: elements between consecutive rows
+  int32_t n1, rs, lead, remap;     // rs: elements between consecutive rows; remap: XCD-aware order
   const uint32_t* bbmask;          // NULL or folded link directions per interior cell
   const uint32_t* bbtab_bits;      // NULL or the folded links with a sign / value table entry
   const uint32_t* bbtab_base;
   const uint32_t* bbtab_src;
   const C* bbtab_sign;
   const C* bbtab_rhs;
+  const uint32_t* bbtab_rel;       // NULL or the table sources relative to the pulling cell
   C omega[NS], one_minus[NS], wshift[NS];
   C cref[NCREF];
   C half_dt, t;
@@ -467,20 +473,15 @@ constexpr int vec_cells() {
 }
 
 // Workgroup order: consecutive workgroups are dealt round-robin to the 8 XCDs of
-// MI355X.  With REMAP the linear workgroup index is renumbered so that each XCD
+// MI355X.  With remap the linear workgroup index is renumbered so that each XCD
 // takes a contiguous range of (mid, outer) rows, whose pulled neighbour rows then
-// hit the same L2 (only when the count is a multiple of 8).  Measured on MI355X:
-// 3 %% faster on the halo path (C4, 512x256x256), 2.5 %% slower on the periodic
-// fold path (256^3), so LBM_XCD_REMAP = 1 renumbers the halo path only.
-#ifndef LBM_XCD_REMAP
-#define LBM_XCD_REMAP 1
-#endif
-template <bool REMAP>
-__device__ __forceinline__ void block_coords(unsigned& bx, unsigned& by, unsigned& bz) {
+// hit the same L2 (only when the count is a multiple of 8).  A run-time choice
+// (lbm_params.xcd_remap; by default on for the halo path, off for the wrap path).
+__device__ __forceinline__ void block_coords(unsigned& bx, unsigned& by, unsigned& bz, int remap) {
   bx = blockIdx.x;
   by = blockIdx.y;
   bz = blockIdx.z;
-  if constexpr (REMAP) {
+  if (remap) {
     const unsigned total = gridDim.x * gridDim.y * gridDim.z;
     if (total %% 8 == 0) {
       const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
@@ -492,14 +493,16 @@ __device__ __forceinline__ void block_coords(unsigned& bx, unsigned& by, unsigne
   }
 }
 
-template <typename S, bool FOLD, int VEC>
+// I: the element index type, int32_t when every index of the array fits it (the
+// usual case: 32-bit offsets from a 64-bit base), int64_t for larger arrays
+template <typename S, bool FOLD, int VEC, typename I>
 __global__ void __launch_bounds__(LBM_BX * LBM_BY)
 fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Codec<S>::C> a) {
   using C = typename Codec<S>::C;
   // full-value arithmetic for FP64 (as pylbm), deviations from the rest state otherwise
   constexpr bool DEV = Codec<S>::kind != LBM_F64;
   unsigned bkx, bky, bkz;
-  block_coords<LBM_XCD_REMAP != 0 && !FOLD>(bkx, bky, bkz);
+  block_coords(bkx, bky, bkz, a.remap);
   const int i = (bkx * blockDim.x + threadIdx.x) * VEC;
   if (i >= a.nin[2]) return;
   const int oo = bkz;
@@ -516,7 +519,7 @@ fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename 
   const int mm = mbase + r;
   if (mm >= a.nin[1]) return;
   const int o = oo + a.halo[0], m = mm + a.halo[1], ii = i + a.halo[2];
-  const int cell = (o * a.n1 + m) * a.rs + a.lead + ii;
+  const I cell = ((I)o * a.n1 + m) * a.rs + a.lead + ii;
 %(loads)s
 %(bbsub)s
 %(gdecl)s
@@ -539,7 +542,7 @@ fused_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename 
   }
 }
 
-template <typename S>
+%(tb_kernel)stemplate <typename S>
 __global__ void __launch_bounds__(256)
 f2m_kernel(const S* __restrict__ f, lbm_layout fl, double* __restrict__ mo, lbm_layout ml,
            const StepArgs<double> a) {
@@ -616,8 +619,8 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   LBM_REQUIRE(L->ns == NS, "layout has ns=%%d, module was generated for %%d", L->ns, NS);
   static const int hv[NS][3] = {%(vel)s};
   for (int k = 0; k < NS; ++k) {
-    a->dst_off[k] = (int32_t)(k * L->pop_stride);
-    a->src_off[k] = (int32_t)(k * L->pop_stride - ((int64_t)hv[k][0] * L->n[1] + hv[k][1]) * L->row_stride - hv[k][2]);
+    a->dst_off[k] = k * L->pop_stride;
+    a->src_off[k] = k * L->pop_stride - ((int64_t)hv[k][0] * L->n[1] + hv[k][1]) * L->row_stride - hv[k][2];
   }
   for (int d = 0; d < 3; ++d) {
     a->nin[d] = (int32_t)(L->n[d] - 2 * L->halo[d]);
@@ -626,7 +629,7 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   a->n1 = (int32_t)L->n[1];
   a->rs = (int32_t)L->row_stride;
   a->lead = (int32_t)L->lead;
-  a->_pad = 0;
+  a->remap = 0;
   for (int k = 0; k < NS; ++k) {
     a->omega[k] = p ? (C)p->omega[k] : (C)0;
     a->one_minus[k] = p ? (C)p->one_minus[k] : (C)1;
@@ -643,6 +646,7 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   a->bbtab_src = p ? p->bbtab_src : nullptr;
   a->bbtab_sign = p ? (const C*)p->bbtab_sign : nullptr;
   a->bbtab_rhs = p ? (const C*)p->bbtab_rhs : nullptr;
+  a->bbtab_rel = p ? p->bbtab_rel : nullptr;
   LBM_REQUIRE(a->bbmask == nullptr || BBSUB_OK, "this stencil has no symmetric pairs: no fused bounce-back");
   return 0;
 }
@@ -663,6 +667,7 @@ static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_p
   // (a 512^2 plane) need the waves of one cell per thread to hide memory latency
   const int64_t cells = (int64_t)a.nin[0] * a.nin[1] * a.nin[2];
   const int vec = (a.nin[2] %% V == 0 && cells / V >= ((int64_t)1 << 20)) ? V : 1;
+  a.remap = (p && p->xcd_remap >= 0) ? (p->xcd_remap != 0) : !fold;
   const int threads = a.nin[2] / vec;
   unsigned bx = LBM_BX;
   while (bx > 64 && (int)(bx / 2) >= threads) bx /= 2;
@@ -674,12 +679,22 @@ static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_p
     for (int d = 0; d < 3; ++d)
       LBM_REQUIRE(a.nin[d] >= a.halo[d], "periodic fold needs extents >= halo");
   }
-  if (vec == V) {
-    if (fold) fused_step<S, true, V><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
-    else fused_step<S, false, V><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+  if (layout_fits_int32(*L)) {
+    if (vec == V) {
+      if (fold) fused_step<S, true, V, int32_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+      else fused_step<S, false, V, int32_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+    } else {
+      if (fold) fused_step<S, true, 1, int32_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+      else fused_step<S, false, 1, int32_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+    }
   } else {
-    if (fold) fused_step<S, true, 1><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
-    else fused_step<S, false, 1><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+    if (vec == V) {
+      if (fold) fused_step<S, true, V, int64_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+      else fused_step<S, false, V, int64_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+    } else {
+      if (fold) fused_step<S, true, 1, int64_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+      else fused_step<S, false, 1, int64_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
+    }
   }
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
@@ -724,7 +739,7 @@ static int eq_m2f_launch(const double* mi, const lbm_layout* ml, void* f, const 
   return 0;
 }
 
-static int same_extents(const lbm_layout* a, const lbm_layout* b) {
+%(tb_launch)sstatic int same_extents(const lbm_layout* a, const lbm_layout* b) {
   LBM_REQUIRE(a && b, "layout is NULL");
   LBM_REQUIRE(b->storage == LBM_F64, "moment layout must be FP64");
   for (int d = 0; d < 3; ++d)
@@ -786,6 +801,43 @@ int lbm_gen_equilibrium(double* d_m, const lbm_layout* ml, void* stream) {
   return 0;
 }
 
+int lbm_gen_run_blocked(void* d_f, void* d_fnew, const lbm_layout* layout, lbm_params* params,
+                        int64_t nsteps, int steps_per_launch, int tile_m, int tile_i, int* parity,
+                        void* stream) {
+  LBM_REQUIRE(params && layout && parity, "NULL argument");
+  LBM_REQUIRE(DIM == 2, "temporal blocking is generated for 2D schemes only (this one is %%dD)", DIM);
+  LBM_REQUIRE(TB_UNIT_VELOCITIES, "temporal blocking needs velocity components in {-1, 0, 1}");
+  LBM_REQUIRE(steps_per_launch >= 1 && steps_per_launch <= TB_MAX_NT, "steps per launch out of [1, %%d]",
+              TB_MAX_NT);
+  LBM_REQUIRE(tile_m >= 1 && tile_i >= 1, "tile sizes must be positive");
+  LBM_REQUIRE(params->bbtab_bits == nullptr || params->bbtab_rel != nullptr,
+              "folded links with table entries need their relative sources (bbtab_rel)");
+  hipStream_t s = (hipStream_t)stream;
+  void* a = d_f;
+  void* b = d_fnew;
+  int64_t done = 0;
+  const int64_t blocks = nsteps / steps_per_launch;
+  for (int64_t it = 0; it < blocks; ++it) {
+    int rc = 0;
+    LBM_DISPATCH_BLOCKED(layout->storage, tb_launch, a, b, layout, params, steps_per_launch, tile_m, tile_i, s, rc);
+    if (rc) return rc;
+    void* tmp = a;
+    a = b;
+    b = tmp;
+    done += steps_per_launch;
+  }
+  for (; done < nsteps; ++done) {
+    int rc = lbm_gen_step(a, b, layout, params, 1, stream);
+    if (rc) return rc;
+    void* tmp = a;
+    a = b;
+    b = tmp;
+    params->t += params->dt;
+  }
+  *parity = (int)(blocks & 1) ^ (int)((nsteps - blocks * steps_per_launch) & 1);
+  return 0;
+}
+
 int lbm_gen_run(void* d_f, void* d_fnew, const lbm_layout* layout, const lbm_velocities* vel,
                 lbm_params* params, const lbm_link_set* links, int nmeth, int64_t nsteps,
                 int ghost, int periodic_fold, int* parity, void* stream) {
@@ -812,6 +864,151 @@ int lbm_gen_run(void* d_f, void* d_fnew, const lbm_layout* layout, const lbm_vel
 }
 
 }  // extern "C"
+"""
+
+
+_TB_KERNEL = r"""// ------------------------------------------------ temporally blocked steps (2D)
+// One workgroup advances a tm x ti tile of the interior (mid x inner axes) by nt
+// steps in LDS: it loads the (tm + 2 nt) x (ti + 2 nt) region around the tile
+// through the periodic wrap of the interior (valid where the wrap path is: a
+// periodic box, or every link list folded into the mask / table), then for
+// s = 1 .. nt computes the cells of the region shrunk by s from the values of
+// step s - 1 with the per-cell arithmetic of fused_step (the same generated
+// lines, so the same bits), and finally stores the tile.  Folded links read their
+// sources at the offsets bbtab_rel gives relative to the pulling cell.
+template <typename C>
+struct TbArgs {
+  int32_t nt, tm, ti, tiles_i;
+  C times[TB_MAX_NT];   // t of each step (the host's sequence of t += dt)
+};
+
+__device__ __forceinline__ int wrap_index(int x, int n) {
+  x %%= n;
+  return x < 0 ? x + n : x;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(TB_THREADS)
+tb_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Codec<S>::C> a,
+        const TbArgs<typename Codec<S>::C> t) {
+  using C = typename Codec<S>::C;
+  constexpr bool DEV = Codec<S>::kind != LBM_F64;
+  extern __shared__ uint4 tb_smem[];
+  const int RW = t.tm + 2 * t.nt, RI = t.ti + 2 * t.nt, RC = RW * RI;
+  S* cur = reinterpret_cast<S*>(tb_smem);
+  S* nxt = cur + NS * RC;
+  const int m0 = (int)(blockIdx.x / t.tiles_i) * t.tm - t.nt;
+  const int i0 = (int)(blockIdx.x %% t.tiles_i) * t.ti - t.nt;
+  const C half_dt = a.half_dt;
+  (void)half_dt;
+  const C* one_minus = a.one_minus;
+  const C* omega = a.omega;
+  const C* cref = a.cref;
+  (void)cref;
+  for (int c = threadIdx.x; c < RC; c += blockDim.x) {
+    const int gm = wrap_index(m0 + c / RI, a.nin[1]), gi = wrap_index(i0 + c %% RI, a.nin[2]);
+    const int64_t src = (int64_t)(a.halo[0] * a.n1 + gm + a.halo[1]) * a.rs + a.lead + gi + a.halo[2];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) cur[k * RC + c] = f[a.dst_off[k] + src];
+  }
+  __syncthreads();
+  for (int s = 1; s <= t.nt; ++s) {
+    const C tt = t.times[s - 1];
+    (void)tt;
+    const int WC = RI - 2 * s, HC = RW - 2 * s;
+    for (int c = threadIdx.x; c < WC * HC; c += blockDim.x) {
+      const int lr = s + c / WC, lc = s + c %% WC;
+      const int lcell = lr * RI + lc;
+      const int gm = wrap_index(m0 + lr, a.nin[1]), gi = wrap_index(i0 + lc, a.nin[2]);
+      const int o = a.halo[0], m = gm + a.halo[1], ii = gi + a.halo[2];
+      (void)o; (void)m; (void)ii;
+      constexpr int j = 0;
+%(tb_loads)s
+%(tb_sub)s
+%(tb_gdecl)s
+      {
+%(tb_coords)s
+%(tb_decode)s
+%(tb_f2m)s
+      if constexpr (DEV) {
+%(tb_coll_dev)s
+%(tb_m2f_dev)s
+%(tb_stores_dev)s
+      } else {
+%(tb_coll)s
+%(tb_m2f)s
+%(tb_stores)s
+      }
+      }
+%(tb_lds_stores)s
+    }
+    __syncthreads();
+    S* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  for (int c = threadIdx.x; c < t.tm * t.ti; c += blockDim.x) {
+    const int lr = t.nt + c / t.ti, lc = t.nt + c %% t.ti;
+    const int gm = m0 + lr, gi = i0 + lc;
+    if (gm >= a.nin[1] || gi >= a.nin[2]) continue;
+    const int64_t dst = (int64_t)(a.halo[0] * a.n1 + gm + a.halo[1]) * a.rs + a.lead + gi + a.halo[2];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) fn[a.dst_off[k] + dst] = cur[k * RC + lr * RI + lc];
+  }
+}
+
+"""
+
+_TB_LAUNCH = r"""// one launch of tb_step: nt steps of every tile; advances params->t by nt dt
+template <typename S>
+static int tb_launch(void* f, void* fn, const lbm_layout* L, lbm_params* p, int nt, int tm, int ti,
+                     hipStream_t s) {
+  using C = typename Codec<S>::C;
+  StepArgs<C> a;
+  int rc = make_args<C>(L, p, &a);
+  if (rc) return rc;
+  LBM_REQUIRE(a.nin[1] >= tm && a.nin[2] >= ti && a.nin[1] >= nt && a.nin[2] >= nt,
+              "tile %%d x %%d / %%d steps larger than the interior", tm, ti, nt);
+  TbArgs<C> t{};
+  t.nt = nt;
+  t.tm = tm;
+  t.ti = ti;
+  t.tiles_i = (a.nin[2] + ti - 1) / ti;
+  for (int q = 0; q < nt; ++q) {
+    t.times[q] = (C)p->t;
+    p->t += p->dt;
+  }
+  const int tiles_m = (a.nin[1] + tm - 1) / tm;
+  const size_t lds = (size_t)2 * NS * (tm + 2 * nt) * (ti + 2 * nt) * sizeof(S);
+  LBM_REQUIRE(lds <= 160 * 1024, "tile needs %%zu bytes of LDS", lds);
+  LBM_CHECK_HIP(hipFuncSetAttribute((const void*)tb_step<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+  tb_step<S><<<dim3((unsigned)(tiles_m * t.tiles_i)), dim3(TB_THREADS), lds, s>>>((const S*)f, (S*)fn, a, t);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+#define LBM_DISPATCH_BLOCKED(kind, FN, f, fn, L, p, nt, tm, ti, s, rc)          \
+  do {                                                                          \
+    switch (kind) {                                                             \
+      case LBM_F64: rc = FN<double>(f, fn, L, p, nt, tm, ti, s); break;         \
+      case LBM_F32: rc = FN<float>(f, fn, L, p, nt, tm, ti, s); break;          \
+      case LBM_F16D: rc = FN<_Float16>(f, fn, L, p, nt, tm, ti, s); break;      \
+      case LBM_BF16D: rc = FN<::lbm::bf16>(f, fn, L, p, nt, tm, ti, s); break;  \
+      case LBM_F32D: rc = FN<::lbm::f32dev>(f, fn, L, p, nt, tm, ti, s); break; \
+      default: ::lbm::set_error("unknown storage kind %%d", (int)(kind)); rc = -1; \
+    }                                                                           \
+  } while (0)
+
+"""
+
+# 3D / 1D schemes: no temporally blocked kernel
+_TB_LAUNCH_NONE = r"""#define LBM_DISPATCH_BLOCKED(kind, FN, f, fn, L, p, nt, tm, ti, s, rc) \
+  do {                                                                \
+    (void)(kind);                                                     \
+    rc = 0;                                                           \
+  } while (0)
+
 """
 
 
@@ -858,7 +1055,7 @@ def generate_source(spec: KernelSpec) -> str:
             if ksym[k] != k:
                 bit = 1 << k
                 bbsub += ["        if (bits & %du) {" % bit,
-                          "          L%d[j] = f[a.dst_off[%d] + cell + j];" % (k, ksym[k]),
+                          "          L%d[j] = f[(I)a.dst_off[%d] + cell + j];" % (k, ksym[k]),
                           "          if (tb & %du) {" % bit,
                           "            using CC = typename Codec<S>::C;",
                           "            const uint32_t e_ = tbase + (uint32_t)__builtin_popcount(tb & %du);" % (bit - 1),
@@ -871,7 +1068,7 @@ def generate_source(spec: KernelSpec) -> str:
     loads = ["S %s;" % ", ".join("L%d[VEC]" % k for k in range(ns))]
     loads.append("if (!FOLD) {")
     for k in range(ns):
-        loads.append("  vload<S, VEC>(L%d, f + (a.src_off[%d] + cell));" % (k, k))
+        loads.append("  vload<S, VEC>(L%d, f + ((I)a.src_off[%d] + cell));" % (k, k))
     loads.append("} else {")
     # wrapped outer / mid coordinates per distinct component value; the contiguous
     # axis is wrapped per element, only for the vectors that straddle the row ends
@@ -886,7 +1083,7 @@ def generate_source(spec: KernelSpec) -> str:
             loads.append("  %s -= (%s >= a.halo[%d] + a.nin[%d]) ? a.nin[%d] : 0;" % (nm, nm, d, d, d))
     for k, (v0, v1, v2) in enumerate(vel):
         loads.append("  {")
-        loads.append("    const int row = a.dst_off[%d] + (%s * a.n1 + %s) * a.rs + a.lead;"
+        loads.append("    const I row = (I)a.dst_off[%d] + ((I)%s * a.n1 + %s) * a.rs + a.lead;"
                      % (k, names[(0, v0)], names[(1, v1)]))
         loads.append("    const int s0 = ii - (%d);" % v2)
         if v2 == 0:
@@ -912,7 +1109,7 @@ def generate_source(spec: KernelSpec) -> str:
     else:
         decode = ["C f%d = DEV ? Codec<S>::decode_dev(L%d[j], a.wshift[%d]) : Codec<S>::decode(L%d[j], a.wshift[%d]);"
                   % (k, k, k, k, k) for k in range(ns)]
-    vstores = ["vstore<S, VEC>(fn + (a.dst_off[%d] + cell), G%d);" % (k, k) for k in range(ns)]
+    vstores = ["vstore<S, VEC>(fn + ((I)a.dst_off[%d] + cell), G%d);" % (k, k) for k in range(ns)]
 
     b1 = _SumBuilder("p")
     f2m_lines = _transform(b1, M, fvars, mvars, ksym, sig_f2m)
@@ -967,7 +1164,52 @@ def generate_source(spec: KernelSpec) -> str:
     coords = ["const C xc%d = a.xorg[%d] + a.dx * (C)%s;" % (d, d, canon_index[3 - dim + d])
               for d in range(dim) if "XYZ"[d] in used]
     vel_txt = ", ".join("{%d, %d, %d}" % v for v in vel)
+
+    # ------------------------------------------- temporally blocked kernel (2D)
+    if dim == 2:
+        tb_loads = ["S %s;" % ", ".join("L%d[1]" % k for k in range(ns))]
+        tb_loads += ["L%d[0] = cur[%d * RC + lcell - (%d * RI + %d)];" % (k, k, vel[k][1], vel[k][2])
+                     for k in range(ns)]
+        tb_sub = []
+        if ksym is not None:
+            tb_sub = ["if (a.bbmask) {",
+                      "  const int64_t mc = (int64_t)gm * a.nin[2] + gi;",
+                      "  const uint32_t bits = a.bbmask[mc];",
+                      "  if (bits) {",
+                      "    const uint32_t tb = (bits & 0x80000000u) ? a.bbtab_bits[mc] : 0u;",
+                      "    const uint32_t tbase = tb ? a.bbtab_base[mc] : 0u;",
+                      "    (void)tbase;"]
+            for k in range(ns):
+                if ksym[k] != k:
+                    bit = 1 << k
+                    tb_sub += ["    if (bits & %du) {" % bit,
+                               "      L%d[0] = cur[%d * RC + lcell];" % (k, ksym[k]),
+                               "      if (tb & %du) {" % bit,
+                               "        using CC = typename Codec<S>::C;",
+                               "        const uint32_t e_ = tbase + (uint32_t)__builtin_popcount(tb & %du);" % (bit - 1),
+                               "        const uint32_t rel = a.bbtab_rel[e_];",
+                               "        const int ks = (int)(rel & 0xffu), dm = (int)((rel >> 8) & 0xffu) - 128,",
+                               "                  di = (int)((rel >> 16) & 0xffu) - 128;",
+                               "        CC r_ = a.bbtab_sign[e_] * Codec<S>::decode_dev(cur[ks * RC + lcell + dm * RI + di], (CC)0);",
+                               "        r_ += a.bbtab_rhs[e_];",
+                               "        L%d[0] = Codec<S>::encode(r_, (CC)0);" % k,
+                               "      }",
+                               "    }"]
+            tb_sub += ["  }", "}"]
+        tb_kernel = _TB_KERNEL % dict(
+            tb_loads=_indent(tb_loads, 6), tb_sub=_indent(tb_sub, 6),
+            tb_gdecl=_indent(["S %s;" % ", ".join("G%d[1]" % k for k in range(ns))], 6),
+            tb_coords=_indent(coords, 6), tb_decode=_indent(decode, 6), tb_f2m=_indent(f2m_block, 6),
+            tb_coll_dev=_indent(coll_dev, 8), tb_m2f_dev=_indent(m2f_dev_block, 8),
+            tb_stores_dev=_indent(stores_dev, 8), tb_coll=_indent(coll, 8), tb_m2f=_indent(m2f_block, 8),
+            tb_stores=_indent(stores, 8),
+            tb_lds_stores=_indent(["nxt[%d * RC + lcell] = G%d[0];" % (k, k) for k in range(ns)], 6))
+        tb_launch = _TB_LAUNCH % {}
+    else:
+        tb_kernel = ""
+        tb_launch = _TB_LAUNCH_NONE
     return _TEMPLATE % dict(
+        tb_kernel=tb_kernel, tb_launch=tb_launch, unit_vel=int(all(abs(c) <= 1 for v in vel for c in v)),
         hash=spec_hash(spec), ns=ns, nc=spec.nconsm, dim=dim, paired=bool(sig_f2m),
         vel=vel_txt,
         loads=_indent(loads), bbsub=_indent(bbsub), bbsub_ok=int(ksym is not None),

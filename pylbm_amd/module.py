@@ -68,7 +68,7 @@ class HipModule:
         """The module of a pylbm_amd Simulation: its generated module, layouts,
         velocities and relaxation parameters (folded link lists are not used)."""
         params = rtm.LbmParams.from_buffer_copy(sim.params)
-        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_src", "bbtab_sign", "bbtab_rhs"):
+        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_src", "bbtab_sign", "bbtab_rhs", "bbtab_rel"):
             setattr(params, name, None)
         return cls(sim.module, sim.container.F.layout, sim.container.m.layout, sim.vel, params, sim.rt)
 

@@ -51,8 +51,9 @@ class LbmParams(ctypes.Structure):
                 ("wshift", ctypes.c_double * MAX_Q), ("cref", ctypes.c_double * MAX_Q),
                 ("dt", ctypes.c_double), ("t", ctypes.c_double), ("bbmask", ctypes.c_void_p),
                 ("bbtab_bits", ctypes.c_void_p), ("bbtab_base", ctypes.c_void_p), ("bbtab_src", ctypes.c_void_p),
-                ("bbtab_sign", ctypes.c_void_p), ("bbtab_rhs", ctypes.c_void_p),
-                ("origin", ctypes.c_double * 3), ("dx", ctypes.c_double)]
+                ("bbtab_sign", ctypes.c_void_p), ("bbtab_rhs", ctypes.c_void_p), ("bbtab_rel", ctypes.c_void_p),
+                ("origin", ctypes.c_double * 3), ("dx", ctypes.c_double),
+                ("xcd_remap", ctypes.c_int32), ("_pad_params", ctypes.c_int32)]
 
 
 class LbmLinkSet(ctypes.Structure):
@@ -146,6 +147,8 @@ GEN_SIGNATURES = {
     "lbm_gen_equilibrium": ([_P, ctypes.POINTER(LbmLayout), _P], _I),
     "lbm_gen_eq_m2f": ([_P, ctypes.POINTER(LbmLayout), _P, ctypes.POINTER(LbmLayout),
                         ctypes.POINTER(LbmParams), _P], _I),
+    "lbm_gen_run_blocked": ([_P, _P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmParams), _I64, _I, _I, _I,
+                             ctypes.POINTER(_I), _P], _I),
     "lbm_gen_run": ([_P, _P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmVelocities),
                      ctypes.POINTER(LbmParams), ctypes.POINTER(LbmLinkSet), _I, _I64, _I, _I,
                      ctypes.POINTER(_I), _P], _I),

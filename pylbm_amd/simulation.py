@@ -147,6 +147,7 @@ class Simulation:
         for d, c in enumerate(self.domain.coords_halo):
             self.params.origin[d] = float(c[0])
         self.params.dx = float(self.domain.dx)
+        self.params.xcd_remap = -1
         vmax = list(self.domain.stencil.vmax)
         # reference state of the FP32-compute formats: conserved moments
         # (rho0, 0, ...), its equilibrium populations w_k rho0
@@ -183,6 +184,9 @@ class Simulation:
         # bounce-back / anti-bounce-back lists folded into the fused step's pulls when
         # that is exact (False: every list runs as a link-list launch; same results)
         self.fold_links = True
+        # small 2D grids on the wrap path advance several steps per launch in LDS
+        # (run() only; same results, see _blocking_shape)
+        self.temporal_blocking = True
         self.init_type = dico.get("inittype", "moments")
         self.init_data = dico.get("init", None)
         self._need_init = True
@@ -254,7 +258,7 @@ class Simulation:
                                                       self.wshift, flat=(st, ld)))
         for d in self._dev_links:
             d.fused = False
-        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_src", "bbtab_sign", "bbtab_rhs"):
+        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_src", "bbtab_sign", "bbtab_rhs", "bbtab_rel"):
             setattr(self.params, name, None)
         if self.fold_links:
             self._fold_links()
@@ -405,6 +409,23 @@ class Simulation:
         wrapped = k * L.pop_stride + (c[0] * L.n[1] + c[1]) * L.row_stride + L.lead + c[2]
         return np.where(ghost, wrapped, src)
 
+    def _relative_sources(self, cell, src):
+        """Table sources relative to their pulling cell, for the temporally blocked run
+        (2D): population | (mid offset + 128) << 8 | (inner offset + 128) << 16, the
+        offsets taken through the periodic wrap; None when a source is farther than
+        one cell away or the scheme is not 2D."""
+        if self.dim != 2 or src.size == 0:
+            return None
+        L, n, h, nin = self._fold_layout()
+        k, o, m, i = _unflat_fn(np.asarray(src, dtype=np.int64), L)
+        cm, ci = np.asarray(cell) // nin[2], np.asarray(cell) % nin[2]
+        dm = (np.asarray(m) - h[1] - cm + nin[1] // 2) % nin[1] - nin[1] // 2
+        di = (np.asarray(i) - h[2] - ci + nin[2] // 2) % nin[2] - nin[2] // 2
+        if np.any(np.abs(dm) > 1) or np.any(np.abs(di) > 1) or np.any(np.asarray(o) != h[0]):
+            return None
+        return (np.asarray(k, np.uint32) | ((dm + 128).astype(np.uint32) << 8)
+                | ((di + 128).astype(np.uint32) << 16)).astype(np.uint32)
+
     def _upload_fold(self, mask, tab, owner):
         """Device copies of the fold mask and table (cell, k, src, sign, rhs) -> params."""
         up = []
@@ -421,8 +442,12 @@ class Simulation:
             mask = mask.copy()
             mask[first_cell] |= np.uint32(1 << 31)   # these cells read their table word
             ctype = np.float64 if self.storage == LBM_F64 else np.float32
-            for name, arr in (("bbtab_bits", bits), ("bbtab_base", base), ("bbtab_src", src.astype(np.uint32)),
-                              ("bbtab_sign", sg.astype(ctype)), ("bbtab_rhs", rh.astype(ctype))):
+            arrays = [("bbtab_bits", bits), ("bbtab_base", base), ("bbtab_src", src.astype(np.uint32)),
+                      ("bbtab_sign", sg.astype(ctype)), ("bbtab_rhs", rh.astype(ctype))]
+            rel = self._relative_sources(cell, src)
+            if rel is not None:
+                arrays.append(("bbtab_rel", rel))
+            for name, arr in arrays:
                 ptr = self.rt.upload(np.ascontiguousarray(arr))
                 setattr(self.params, name, ptr)
                 up.append(ptr)
@@ -569,6 +594,18 @@ class Simulation:
         c = self.container
         self.params.t = self.t
         parity = ctypes.c_int(0)
+        tb = self._blocking_shape() if fold else None
+        if tb is not None and nsteps >= tb[0]:
+            nt, tm, ti = tb
+            self._check(self.module.lbm_gen_run_blocked(ctypes.c_void_p(c.F.ptr), ctypes.c_void_p(c.Fnew.ptr),
+                                                        ctypes.byref(c.F.layout), ctypes.byref(self.params),
+                                                        int(nsteps), nt, tm, ti, ctypes.byref(parity),
+                                                        self.rt.stream), "run (blocked)")
+            if parity.value:
+                c.F, c.Fnew = c.Fnew, c.F
+            self.nt += nsteps
+            self.t = self.t + nsteps * self.dt
+            return
         ghost = 0 if fold else self._ghost_mode
         nmeth = 0 if fold else len(self._active_links)
         self._check(self.module.lbm_gen_run(ctypes.c_void_p(c.F.ptr), ctypes.c_void_p(c.Fnew.ptr),
@@ -580,6 +617,34 @@ class Simulation:
             c.F, c.Fnew = c.Fnew, c.F
         self.nt += nsteps
         self.t = self.t + nsteps * self.dt
+
+    def _blocking_shape(self):
+        """(steps per launch, tile rows, tile columns) of the temporally blocked run, or
+        None: 2D schemes on the wrap path (periodic, or every list folded with its
+        table sources within one cell) whose grid is small enough that a step does not
+        fill the GPU (at most 2^21 cells).  The tile and its halo live in LDS, two
+        copies of ns populations per cell within 144 KiB."""
+        if not self.temporal_blocking or self.dim != 2 or not self._wrap:
+            return None
+        if np.any(np.abs(np.asarray(self.scheme.stencil.get_all_velocities())) > 1):
+            return None
+        if self.params.bbtab_bits and not self.params.bbtab_rel:
+            return None
+        nin = [int(x) for x in self.domain.shape_in]
+        if nin[0] * nin[1] > (1 << 21):
+            return None
+        es = self.container.F.layout.elem_size
+        rmax = int(math.sqrt(144 * 1024 / (2 * self.ns * es)))
+        for nt in ((8, 6, 4) if es != 4 else (6, 8, 4)):
+            t = (rmax - 2 * nt) // 8 * 8
+            if t >= 8:
+                break
+        else:
+            return None
+        nt = min(nt, min(nin))
+        if nt < 2:
+            return None
+        return nt, min(t, nin[0]), min(t, nin[1])
 
     def sync(self):
         self.rt.sync()

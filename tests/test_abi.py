@@ -95,3 +95,21 @@ def test_module_object_uses_pylbm_argument_names():
     }
     for name, args in expect.items():
         assert inspect.getfullargspec(getattr(HipModule, name)).args == ["self"] + args, name
+
+
+def test_layout_beyond_int32_accepted():
+    """Arrays of 2^31 elements and more are accepted (64-bit element indices): a
+    512^3 D3Q19 FP32 population array has about 2.8e9 elements.  lbm_reduce_blocks
+    validates the layout without touching the GPU."""
+    import ctypes
+    from pylbm_amd import runtime as rtm
+    from pylbm_amd.storage import layout_elements, make_layout
+    lib = rtm.load_core_library()
+    L = make_layout(19, (514, 514, 514), (1, 1, 1), rtm.LBM_F32, interleaved=True)
+    assert layout_elements(L) > (1 << 31)
+    nb = ctypes.c_int64(0)
+    assert lib.lbm_reduce_blocks(ctypes.byref(L), ctypes.byref(nb)) == 0, lib.lbm_last_error()
+    assert nb.value == 1024
+    P = make_layout(19, (514, 514, 514), (1, 1, 1), rtm.LBM_F64, interleaved=False)
+    assert layout_elements(P) > (1 << 31)
+    assert lib.lbm_reduce_blocks(ctypes.byref(P), ctypes.byref(nb)) == 0, lib.lbm_last_error()
