@@ -269,7 +269,11 @@ typedef struct lbm_params {
    * of rows (their shared neighbour rows then meet in one L2), 0 = the hardware's
    * round-robin order, -1 = the default (1 on the halo path, 0 on the wrap path) */
   int32_t xcd_remap;
-  int32_t _pad_params;
+  /* cells per thread of the step along the contiguous axis: 0 = the default (4
+   * for the 2- and 4-byte formats where that leaves >= 2^20 threads, else 1),
+   * 1 = one cell per thread, 4 = four (the contiguous interior extent must be a
+   * multiple of 4); every choice gives the same bits */
+  int32_t vec_cells;
 } lbm_params;
 
 /* information on the generated module (ns, nconsm, dim) */

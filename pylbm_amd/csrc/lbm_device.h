@@ -193,6 +193,52 @@ __device__ __forceinline__ void vload(S (&out)[V], const S* p) {
   __builtin_memcpy(out, p, sizeof(S) * V);
 }
 
+// Pulled vector of V populations at p[D], p[D + 1], ..., p[D + V - 1], where p is the
+// vector-aligned element of the cell (element index a multiple of V) and D the
+// compile-time pull offset along the contiguous axis (minus the velocity's inner
+// component).  4- and 8-byte formats load p + D directly (dword-aligned for every
+// D).  For the 2-byte formats with V = 2 or 4 the element p + D is only 2-byte
+// aligned when D is odd, and such a load would be split into 16-bit loads: the
+// vector is then assembled from aligned dword loads around it, joined with
+// v_alignbyte_b32 (bytes 2-5 of the 8 bytes {hi, lo}).  Every element read lies
+// within one element of [p + D, p + D + V - 1].  That stays inside the row:
+// |D| <= halo, and where the vector reaches the outer end of a halo (|D| = halo,
+// odd) the row's lead before it and its padding after it are not empty (the lead
+// aligns the interior start, lead + halo = 0 mod 64 elements, and the pitch rounds
+// lead + 2 halo + interior, interior = 0 mod V, up to a multiple of 64).
+template <typename S, int V, int D>
+__device__ __forceinline__ void vload_pull(S (&out)[V], const S* p) {
+  if constexpr (sizeof(S) != 2 || (V != 2 && V != 4) || ((D % V) + V) % V == 0) {
+    vload<S, V>(out, p + D);
+  } else if constexpr (V == 2) {
+    const uint32_t* w = (const uint32_t*)(p + (D - 1));
+    const uint32_t o = __builtin_amdgcn_alignbyte(w[1], w[0], 2);
+    __builtin_memcpy(&out[0], &o, 4);
+  } else {
+    constexpr int R = ((D % 4) + 4) % 4;
+    uint32_t o0, o1;
+    if constexpr (R == 2) {
+      const uint32_t* w = (const uint32_t*)(p + D);
+      o0 = w[0];
+      o1 = w[1];
+    } else if constexpr (R == 1) {
+      const uint32_t* w = (const uint32_t*)(p + (D - 1));
+      const uint2 A = *(const uint2*)w;
+      const uint32_t b0 = w[2];
+      o0 = __builtin_amdgcn_alignbyte(A.y, A.x, 2);
+      o1 = __builtin_amdgcn_alignbyte(b0, A.y, 2);
+    } else {
+      const uint32_t* w = (const uint32_t*)(p + (D + 1));
+      const uint2 A = *(const uint2*)w;
+      const uint32_t pm = w[-1];
+      o0 = __builtin_amdgcn_alignbyte(A.x, pm, 2);
+      o1 = __builtin_amdgcn_alignbyte(A.y, A.x, 2);
+    }
+    __builtin_memcpy(&out[0], &o0, 4);
+    __builtin_memcpy(&out[2], &o1, 4);
+  }
+}
+
 template <typename S, int V>
 __device__ __forceinline__ void vstore(S* p, const S (&in)[V]) {
   using B = typename Bits<S>::T;

@@ -460,16 +460,23 @@ struct StepArgs {
 #define LBM_MROWS 1
 #endif
 
-// Cells per thread along the contiguous axis (vector loads / stores): 0 = by
-// element size, measured on MI355X: 4 cells (16 B per lane) for the 4-byte
-// formats, 1 for FP64 (no gain) and for the 16-bit formats (their pulled rows
-// are only 2-byte aligned and the vector loads split).
+// Cells per thread along the contiguous axis of the vector instantiation (vector
+// loads / stores): 0 = by element size: 4 cells for the 4-byte formats (16 B per
+// lane) and the 2-byte formats (8 B per lane, pulled rows at odd offsets assembled
+// by vload_pull), 1 for FP64 (measured no gain on MI355X).
 #ifndef LBM_VEC
 #define LBM_VEC 0
 #endif
+#ifndef LBM_FORCE_I64
+#define LBM_FORCE_I64 0
+#endif
+// whether the launch picks the vector instantiation by itself for the 2-byte formats
+#ifndef LBM_VEC16_AUTO
+#define LBM_VEC16_AUTO 0
+#endif
 template <typename S>
 constexpr int vec_cells() {
-  return LBM_VEC > 0 ? LBM_VEC : (sizeof(S) == 4 ? 4 : 1);
+  return LBM_VEC > 0 ? LBM_VEC : (sizeof(S) <= 4 ? 4 : 1);
 }
 
 // Workgroup order: consecutive workgroups are dealt round-robin to the 8 XCDs of
@@ -666,7 +673,11 @@ static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_p
   // V cells per thread only where that still leaves >= 2^20 threads: small grids
   // (a 512^2 plane) need the waves of one cell per thread to hide memory latency
   const int64_t cells = (int64_t)a.nin[0] * a.nin[1] * a.nin[2];
-  const int vec = (a.nin[2] %% V == 0 && cells / V >= ((int64_t)1 << 20)) ? V : 1;
+  const int want = p ? p->vec_cells : 0;
+  LBM_REQUIRE(want == 0 || want == 1 || (want == V && a.nin[2] %% V == 0),
+              "vec_cells %%d not available (%%d cells per thread, contiguous extent %%d)", want, V, a.nin[2]);
+  const bool auto_vec = sizeof(S) != 2 || LBM_VEC16_AUTO;
+  const int vec = want ? want : ((auto_vec && a.nin[2] %% V == 0 && cells / V >= ((int64_t)1 << 20)) ? V : 1);
   a.remap = (p && p->xcd_remap >= 0) ? (p->xcd_remap != 0) : !fold;
   const int threads = a.nin[2] / vec;
   unsigned bx = LBM_BX;
@@ -679,7 +690,7 @@ static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_p
     for (int d = 0; d < 3; ++d)
       LBM_REQUIRE(a.nin[d] >= a.halo[d], "periodic fold needs extents >= halo");
   }
-  if (layout_fits_int32(*L)) {
+  if (layout_fits_int32(*L) && !LBM_FORCE_I64) {
     if (vec == V) {
       if (fold) fused_step<S, true, V, int32_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
       else fused_step<S, false, V, int32_t><<<grid, block, 0, s>>>((const S*)f, (S*)fn, a);
@@ -1068,7 +1079,7 @@ def generate_source(spec: KernelSpec) -> str:
     loads = ["S %s;" % ", ".join("L%d[VEC]" % k for k in range(ns))]
     loads.append("if (!FOLD) {")
     for k in range(ns):
-        loads.append("  vload<S, VEC>(L%d, f + ((I)a.src_off[%d] + cell));" % (k, k))
+        loads.append("  vload_pull<S, VEC, %d>(L%d, f + ((I)a.src_off[%d] + (%d) + cell));" % (-vel[k][2], k, k, vel[k][2]))
     loads.append("} else {")
     # wrapped outer / mid coordinates per distinct component value; the contiguous
     # axis is wrapped per element, only for the vectors that straddle the row ends
@@ -1090,7 +1101,7 @@ def generate_source(spec: KernelSpec) -> str:
             loads.append("    vload<S, VEC>(L%d, f + (row + s0));" % k)
         else:
             loads.append("    if (s0 >= a.halo[2] && s0 + VEC <= a.halo[2] + a.nin[2]) {")
-            loads.append("      vload<S, VEC>(L%d, f + (row + s0));" % k)
+            loads.append("      vload_pull<S, VEC, %d>(L%d, f + (row + ii));" % (-v2, k))
             loads.append("    } else {")
             loads.append("      for (int j = 0; j < VEC; ++j) {")
             loads.append("        int c = s0 + j;")

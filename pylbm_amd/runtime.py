@@ -53,7 +53,7 @@ class LbmParams(ctypes.Structure):
                 ("bbtab_bits", ctypes.c_void_p), ("bbtab_base", ctypes.c_void_p), ("bbtab_src", ctypes.c_void_p),
                 ("bbtab_sign", ctypes.c_void_p), ("bbtab_rhs", ctypes.c_void_p), ("bbtab_rel", ctypes.c_void_p),
                 ("origin", ctypes.c_double * 3), ("dx", ctypes.c_double),
-                ("xcd_remap", ctypes.c_int32), ("_pad_params", ctypes.c_int32)]
+                ("xcd_remap", ctypes.c_int32), ("vec_cells", ctypes.c_int32)]
 
 
 class LbmLinkSet(ctypes.Structure):

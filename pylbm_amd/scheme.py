@@ -38,6 +38,16 @@ REL_U = sp.symbols("rel_ux, rel_uy, rel_uz")   # pylbm's relative-velocity symbo
 RU = sp.symbols("ru0, ru1, ru2")               # their names in the generated code
 
 
+def _exact_or_double(x) -> Fraction:
+    """An entry of M or M^-1 as a Fraction: exact when it is rational, else the
+    correctly rounded double of its value (a scheme velocity such as 10 sqrt(3) makes
+    the entries irrational; pylbm computes them in floating point)."""
+    x = sp.sympify(x)
+    if x.is_Rational:
+        return Fraction(int(x.p), int(x.q))
+    return Fraction(float(sp.Float(x.evalf(40), 40)))
+
+
 def _fixed_point_subs(expr, pairs):
     expr = sp.sympify(expr)
     for _ in range(max(1, len(pairs))):
@@ -215,8 +225,7 @@ class Scheme:
             ren = list(zip(REL_U, RU))
             Tus = [[sp.nsimplify(self.Tu[i, j]).xreplace(dict(ren)) for j in range(ns)] for i in range(ns)]
             Tmus = [[sp.nsimplify(self.Tmu[i, j]).xreplace(dict(ren)) for j in range(ns)] for i in range(ns)]
-        frac = lambda mat: [[Fraction(int(sp.fraction(mat[i, j])[0]), int(sp.fraction(mat[i, j])[1]))
-                             for j in range(ns)] for i in range(ns)]
+        frac = lambda mat: [[_exact_or_double(mat[i, j]) for j in range(ns)] for i in range(ns)]
         return KernelSpec(dim=self.dim, ns=ns, nconsm=nc,
                           velocities=[tuple(int(c) for c in v) for v in self.stencil.get_all_velocities()],
                           M=frac(self.M), invM=frac(self.invM), eq=eq, s=s, sources=sources, msyms=msyms,

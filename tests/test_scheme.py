@@ -90,3 +90,21 @@ def test_orthogonal_d3q19():
     G = M @ M.T
     assert np.allclose(G, np.diag(np.diag(G)))
     assert int(np.count_nonzero(M)) == 213
+
+
+def test_irrational_scheme_velocity_kept():
+    """A scheme velocity that sympy recognises as irrational (pylbm's Kelvin-Helmholtz
+    demo: la = sqrt(3) / 0.1 = 10 sqrt(3)) keeps its value in the device matrices: M
+    and M^-1 of the kernel spec equal the oracle's floating-point matrices (a
+    truncation to the integer 17 would leave them 1.8 % off), and their product is the
+    identity to rounding."""
+    from pylbm_amd import demos
+    d, _ = demos.kelvin_helmholtz(1.0 / 64)
+    spec = Scheme(d).kernel_spec()
+    o = OScheme(d)
+    M = np.array([[float(c) for c in r] for r in spec.M])
+    iM = np.array([[float(c) for c in r] for r in spec.invM])
+    assert np.allclose(M, np.asarray(o.Mnum, float), rtol=1e-15, atol=0)
+    assert np.allclose(iM, np.asarray(o.invMnum, float), rtol=1e-13, atol=1e-18)
+    assert M[1, 1] == pytest.approx(np.sqrt(3) / 0.1, rel=1e-15)
+    assert np.abs(M @ iM - np.eye(spec.ns)).max() < 1e-13
