@@ -423,8 +423,11 @@ using namespace lbm;
 #define NCREF (NCONSM > 0 ? NCONSM : 1)
 #define DIM %(dim)d
 #define TB_UNIT_VELOCITIES %(unit_vel)d  // every velocity component in {-1, 0, 1}
-#define TB_THREADS 512                 // threads of a temporally blocked workgroup
+#ifndef TB_THREADS
+#define TB_THREADS 1024                // threads of a temporally blocked workgroup
+#endif
 #define TB_MAX_NT 16                   // steps per temporally blocked launch, at most
+#define TB_MASK %(bbsub_ok)d                    // folded links possible: mask words kept in LDS
 
 namespace {
 
@@ -893,9 +896,13 @@ struct TbArgs {
   C times[TB_MAX_NT];   // t of each step (the host's sequence of t += dt)
 };
 
-__device__ __forceinline__ int wrap_index(int x, int n) {
-  x %%= n;
-  return x < 0 ? x + n : x;
+// x in [-n, 3n) -> x mod n, without a division (tile regions reach at most nt < n
+// cells before the interior and less than 2n + nt after its start)
+__device__ __forceinline__ int wrap_near(int x, int n) {
+  x += (x < 0) ? n : 0;
+  x -= (x >= n) ? n : 0;
+  x -= (x >= n) ? n : 0;
+  return x;
 }
 
 template <typename S>
@@ -908,6 +915,14 @@ tb_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Cod
   const int RW = t.tm + 2 * t.nt, RI = t.ti + 2 * t.nt, RC = RW * RI;
   S* cur = reinterpret_cast<S*>(tb_smem);
   S* nxt = cur + NS * RC;
+  // constant over the launch: the region's mask words (folded links) and, with
+  // table links, every region cell's table bits and first entry (loaded with the
+  // populations, so a step reads its entries after one round trip instead of two)
+  uint32_t* msk = reinterpret_cast<uint32_t*>(cur + 2 * NS * RC);
+  uint32_t* tbits = msk + RC;
+  uint32_t* tbase_l = tbits + RC;
+  const bool table = TB_MASK && a.bbmask && a.bbtab_bits;
+  (void)msk; (void)tbits; (void)tbase_l; (void)table;
   const int m0 = (int)(blockIdx.x / t.tiles_i) * t.tm - t.nt;
   const int i0 = (int)(blockIdx.x %% t.tiles_i) * t.ti - t.nt;
   const C half_dt = a.half_dt;
@@ -917,20 +932,39 @@ tb_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Cod
   const C* cref = a.cref;
   (void)cref;
   for (int c = threadIdx.x; c < RC; c += blockDim.x) {
-    const int gm = wrap_index(m0 + c / RI, a.nin[1]), gi = wrap_index(i0 + c %% RI, a.nin[2]);
+    const int lr = c / RI;
+    const int gm = wrap_near(m0 + lr, a.nin[1]), gi = wrap_near(i0 + c - lr * RI, a.nin[2]);
     const int64_t src = (int64_t)(a.halo[0] * a.n1 + gm + a.halo[1]) * a.rs + a.lead + gi + a.halo[2];
 #pragma unroll
     for (int k = 0; k < NS; ++k) cur[k * RC + c] = f[a.dst_off[k] + src];
+    if (TB_MASK && a.bbmask) {
+      const int64_t mc = (int64_t)gm * a.nin[2] + gi;
+      msk[c] = a.bbmask[mc];
+      if (table) {
+        tbits[c] = a.bbtab_bits[mc];
+        tbase_l[c] = a.bbtab_base[mc];
+      }
+    }
   }
   __syncthreads();
   for (int s = 1; s <= t.nt; ++s) {
     const C tt = t.times[s - 1];
     (void)tt;
     const int WC = RI - 2 * s, HC = RW - 2 * s;
+    // the cells of the shrunk region in row-major order, blockDim.x apart: the
+    // (row, column) of the next one follow by adding (dq, dr) with a carry
+    const int dq = (int)blockDim.x / WC, dr = (int)blockDim.x - dq * WC;
+    int rr = (int)threadIdx.x / WC, qq = (int)threadIdx.x - rr * WC;
     for (int c = threadIdx.x; c < WC * HC; c += blockDim.x) {
-      const int lr = s + c / WC, lc = s + c %% WC;
+      const int lr = s + rr, lc = s + qq;
+      qq += dr;
+      rr += dq;
+      if (qq >= WC) {
+        qq -= WC;
+        ++rr;
+      }
       const int lcell = lr * RI + lc;
-      const int gm = wrap_index(m0 + lr, a.nin[1]), gi = wrap_index(i0 + lc, a.nin[2]);
+      const int gm = wrap_near(m0 + lr, a.nin[1]), gi = wrap_near(i0 + lc, a.nin[2]);
       const int o = a.halo[0], m = gm + a.halo[1], ii = gi + a.halo[2];
       (void)o; (void)m; (void)ii;
       constexpr int j = 0;
@@ -990,7 +1024,9 @@ static int tb_launch(void* f, void* fn, const lbm_layout* L, lbm_params* p, int 
     p->t += p->dt;
   }
   const int tiles_m = (a.nin[1] + tm - 1) / tm;
-  const size_t lds = (size_t)2 * NS * (tm + 2 * nt) * (ti + 2 * nt) * sizeof(S);
+  const size_t region = (size_t)(tm + 2 * nt) * (ti + 2 * nt);
+  const bool mask = TB_MASK && p->bbmask, table = mask && p->bbtab_bits;
+  const size_t lds = region * (2 * NS * sizeof(S) + (mask ? 4 : 0) + (table ? 8 : 0));
   LBM_REQUIRE(lds <= 160 * 1024, "tile needs %%zu bytes of LDS", lds);
   LBM_CHECK_HIP(hipFuncSetAttribute((const void*)tb_step<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
@@ -1183,30 +1219,36 @@ def generate_source(spec: KernelSpec) -> str:
                      for k in range(ns)]
         tb_sub = []
         if ksym is not None:
+            # bounce-back copies as selects of unconditional LDS reads (one branch per
+            # cell instead of one per direction), then the table links: bits and
+            # first entry from LDS, the entries from memory
             tb_sub = ["if (a.bbmask) {",
-                      "  const int64_t mc = (int64_t)gm * a.nin[2] + gi;",
-                      "  const uint32_t bits = a.bbmask[mc];",
-                      "  if (bits) {",
-                      "    const uint32_t tb = (bits & 0x80000000u) ? a.bbtab_bits[mc] : 0u;",
-                      "    const uint32_t tbase = tb ? a.bbtab_base[mc] : 0u;",
-                      "    (void)tbase;"]
+                      "  const uint32_t bits = msk[lcell];",
+                      "  if (bits) {"]
+            for k in range(ns):
+                if ksym[k] != k:
+                    tb_sub += ["    {",
+                               "      const S b_ = cur[%d * RC + lcell];" % ksym[k],
+                               "      L%d[0] = (bits & %du) ? b_ : L%d[0];" % (k, 1 << k, k),
+                               "    }"]
+            tb_sub += ["    if (bits & 0x80000000u) {",
+                       "      using CC = typename Codec<S>::C;",
+                       "      const uint32_t tb = tbits[lcell], tbase = tbase_l[lcell];"]
             for k in range(ns):
                 if ksym[k] != k:
                     bit = 1 << k
-                    tb_sub += ["    if (bits & %du) {" % bit,
-                               "      L%d[0] = cur[%d * RC + lcell];" % (k, ksym[k]),
-                               "      if (tb & %du) {" % bit,
-                               "        using CC = typename Codec<S>::C;",
+                    tb_sub += ["      if (tb & %du) {" % bit,
                                "        const uint32_t e_ = tbase + (uint32_t)__builtin_popcount(tb & %du);" % (bit - 1),
                                "        const uint32_t rel = a.bbtab_rel[e_];",
+                               "        const CC sg_ = a.bbtab_sign[e_];",
+                               "        const CC rh_ = a.bbtab_rhs[e_];",
                                "        const int ks = (int)(rel & 0xffu), dm = (int)((rel >> 8) & 0xffu) - 128,",
                                "                  di = (int)((rel >> 16) & 0xffu) - 128;",
-                               "        CC r_ = a.bbtab_sign[e_] * Codec<S>::decode_dev(cur[ks * RC + lcell + dm * RI + di], (CC)0);",
-                               "        r_ += a.bbtab_rhs[e_];",
+                               "        CC r_ = sg_ * Codec<S>::decode_dev(cur[ks * RC + lcell + dm * RI + di], (CC)0);",
+                               "        r_ += rh_;",
                                "        L%d[0] = Codec<S>::encode(r_, (CC)0);" % k,
-                               "      }",
-                               "    }"]
-            tb_sub += ["  }", "}"]
+                               "      }"]
+            tb_sub += ["    }", "  }", "}"]
         tb_kernel = _TB_KERNEL % dict(
             tb_loads=_indent(tb_loads, 6), tb_sub=_indent(tb_sub, 6),
             tb_gdecl=_indent(["S %s;" % ", ".join("G%d[1]" % k for k in range(ns))], 6),

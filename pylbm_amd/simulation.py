@@ -622,8 +622,14 @@ class Simulation:
         """(steps per launch, tile rows, tile columns) of the temporally blocked run, or
         None: 2D schemes on the wrap path (periodic, or every list folded with its
         table sources within one cell) whose grid is small enough that a step does not
-        fill the GPU (at most 2^21 cells).  The tile and its halo live in LDS, two
-        copies of ns populations per cell within 144 KiB."""
+        fill the GPU (at most 2^21 cells).  The tile and its halo live in LDS: two
+        copies of ns populations per cell, plus the mask word with folded links and
+        the table bits and first entry with table links, within the 160 KiB of a
+        CU.  The preferred shapes were measured on MI355X
+        (D2Q9, 128^2 to 1024 x 256): 6 steps of 32 x 32 tiles for the 4- and 2-byte
+        formats, 4 steps of 16 x 32 tiles for FP64, 6 steps of 8 x 16 tiles for FP64
+        grids of at most 2^15 cells (more tiles than CUs matter more there than the
+        recomputed halo); larger schemes shrink the tile until it fits."""
         if not self.temporal_blocking or self.dim != 2 or not self._wrap:
             return None
         if np.any(np.abs(np.asarray(self.scheme.stencil.get_all_velocities())) > 1):
@@ -631,20 +637,29 @@ class Simulation:
         if self.params.bbtab_bits and not self.params.bbtab_rel:
             return None
         nin = [int(x) for x in self.domain.shape_in]
-        if nin[0] * nin[1] > (1 << 21):
+        cells = nin[0] * nin[1]
+        if cells > (1 << 21):
             return None
         es = self.container.F.layout.elem_size
-        rmax = int(math.sqrt(144 * 1024 / (2 * self.ns * es)))
-        for nt in ((8, 6, 4) if es != 4 else (6, 8, 4)):
-            t = (rmax - 2 * nt) // 8 * 8
-            if t >= 8:
-                break
+        table = bool(self.params.bbmask and self.params.bbtab_bits)
+        per_cell = 2 * self.ns * es + (4 if self.params.bbmask else 0) + (8 if table else 0)
+        if es == 8:
+            nt, tm, ti = (6, 8, 16) if cells <= (1 << 15) else (4, 16, 32)
         else:
-            return None
+            nt, tm, ti = 6, 32, 32
+        while (tm + 2 * nt) * (ti + 2 * nt) * per_cell > 160 * 1024:
+            if ti > 16:
+                ti //= 2
+            elif tm > 8:
+                tm //= 2
+            elif nt > 2:
+                nt -= 1
+            else:
+                return None
         nt = min(nt, min(nin))
         if nt < 2:
             return None
-        return nt, min(t, nin[0]), min(t, nin[1])
+        return nt, min(tm, nin[0]), min(ti, nin[1])
 
     def sync(self):
         self.rt.sync()

@@ -653,6 +653,16 @@ def _plain_bounce_back(d):
     return d
 
 
+def _all_walls_moving(n):
+    """The lid cavity with every wall carrying the lid's value: table links all round,
+    more of them per tile region than the blocked kernel caches in LDS (the small box
+    is wrapped several times into one region), so the entries read from memory are
+    exercised too."""
+    d = cases.lid_cavity_d2q9(n)
+    d["box"]["label"] = [1, 1, 1, 1]
+    return d
+
+
 _TB_CASES = {
     "lid": lambda: cases.lid_cavity_d2q9(40),
     "karman": lambda: cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0)),
@@ -661,6 +671,8 @@ _TB_CASES = {
     "forced_small": lambda: cases.forced_periodic_d2q9(),
     "relvel_lid": lambda: _plain_bounce_back(__import__("pylbm_amd.demos", fromlist=["x"]).lid_driven_cavity(1.0 / 32)[0]),
     "vectorial": lambda: cases.d2q5q9_vectorial(24),
+    "moving_walls": lambda: _all_walls_moving(12),
+    "moving_walls_large": lambda: _all_walls_moving(64),
 }
 
 
@@ -677,7 +689,8 @@ def test_temporal_blocking_bit_identical(hip, name, dtype):
     b = Simulation(d, dtype=dtype)
     b.temporal_blocking = False
     shape = a._blocking_shape()
-    if name in ("lid", "decaying", "forced", "forced_small", "relvel_lid", "vectorial"):
+    if name in ("lid", "decaying", "forced", "forced_small", "relvel_lid", "vectorial", "moving_walls",
+                "moving_walls_large"):
         assert shape is not None, name
     if shape is None:
         pytest.skip("no temporally blocked run for this case")
