@@ -315,6 +315,7 @@ class LatticeState:
         rows = [[float(v) for v in M[i]] for i in (self.irho, self.iqx, self.iqy)]
         self._mc = (ctypes.c_double * (3 * self.sim.ns))(*[v for r in rows for v in r])
         self._direct = self.sim.storage == rtm.LBM_F32
+        self.blocked = True
 
     def read_state(self, d_state):
         """Conserved moments of the current populations into an FP32 state buffer
@@ -334,8 +335,8 @@ class LatticeState:
                                                         self.rt.stream), "lbm_sur_moments_to_state")
 
     def _bind(self):
-        """ctypes arguments of the judge path, built once (the buffers of F, Fnew and m
-        do not move; F / Fnew swap only for odd step counts, which run() forbids)."""
+        """ctypes arguments of the judge path, built again whenever F and Fnew have
+        exchanged their buffers (an odd number of launches)."""
         sim = self.sim
         c = sim.container
         m = c.m
@@ -362,8 +363,11 @@ class LatticeState:
         sim._update_m = True
 
     def run(self, nsteps):
-        """nsteps plain steps (fully periodic: the folded fused kernel, no links)."""
-        if nsteps % 2:
+        """nsteps plain steps (fully periodic, no links): the temporally blocked run
+        (several steps per launch in LDS, bit-identical to one launch per step) when
+        ``blocked`` is set and the grid allows it, else one folded fused launch per
+        step."""
+        if nsteps % 2 or (self.blocked and self.sim._blocking_shape() is not None):
             self.sim.run(nsteps)
             return
         sim = self.sim
