@@ -307,6 +307,19 @@ int lbm_gen_equilibrium(double* d_m, const lbm_layout* mlayout, void* stream);
  * initialisation of pylbm/simulation.py:314-316 */
 int lbm_gen_eq_m2f(const double* d_m, const lbm_layout* mlayout, void* d_f,
                    const lbm_layout* flayout, const lbm_params* params, void* stream);
+/* pylbm's split routines, as Simulation.transport / relaxation / source_term call
+ * them (pylbm/simulation.py:322-342):
+ * transport: d_fnew[k, x] = d_f[k, x - v_k] on the interior (a copy of the stored
+ * words; the ghost slots it reads are the caller's). pylbm/algorithm/base.py:273-296 */
+int lbm_gen_transport(const void* d_f, void* d_fnew, const lbm_layout* flayout,
+                      const lbm_params* params, void* stream);
+/* relaxation in place on FP64 moments over the halo-inclusive domain,
+ * m_i <- (1 - s_i) m_i + s_i eq_i (params->omega / one_minus). pylbm/algorithm/base.py:404-437 */
+int lbm_gen_relaxation(double* d_m, const lbm_layout* mlayout, const lbm_params* params, void* stream);
+/* source terms in place on FP64 moments over the interior: one explicit Euler half
+ * step m_c <- m_c + dt/2 S(m, X, Y, Z, t) (params->dt, t, origin, dx).
+ * pylbm/algorithm/base.py:439-504, pylbm/algorithm/ode.py:11-16 */
+int lbm_gen_source_term(double* d_m, const lbm_layout* mlayout, const lbm_params* params, void* stream);
 
 /*
  * nsteps complete time steps (ghost fill, link lists in order, fused step,

@@ -255,6 +255,35 @@ class Simulation:
         for idx, fn in self.scheme.sources:
             m[idx] = m[idx] + 0.5 * self.dt * fn(list(m), self.t, *coords)
 
+    # ------------------------------------------------------------ split routines
+    def transport(self, f=None, fnew=None):
+        """fnew[k, x] = f[k, x - v_k] on the interior, the ghost slots of f as they are
+        (pylbm/algorithm/base.py:273-296; Simulation.transport, simulation.py:322-327)."""
+        f = self.F if f is None else f
+        fnew = self.Fnew if fnew is None else fnew
+        for k in range(self.ns):
+            fnew[(k,) + self._interior()[1:]] = f[(k,) + self._interior(self.v[k])[1:]]
+
+    def relaxation(self, m):
+        """m_i <- (1 - s_i) m_i + s_i eq_i(m) on every cell of the moment array, in place
+        (pylbm/algorithm/base.py:404-437; the routine relaxes towards eq itself, also for
+        a relative-velocity scheme)."""
+        eq = self._eq_full(m)
+        for i in range(self.ns):
+            s = self.scheme.s[i]
+            if s != 0:
+                m[i] = (1 - s) * m[i] + s * eq[i]
+
+    def source_term(self, m):
+        """One explicit Euler half step m_c <- m_c + dt/2 S(m, X, t) on the interior cells
+        of the halo-inclusive moment array, in place (pylbm/algorithm/base.py:439-504,
+        algorithm/ode.py:11-16)."""
+        if not self.scheme.sources:
+            return
+        inner = m[self._interior()]
+        self._sources(inner)
+        m[self._interior()] = inner
+
     def one_time_step(self):
         self.ghost_fill(self.F)
         for meth in self.methods:

@@ -594,6 +594,60 @@ equilibrium_kernel(double* __restrict__ mo, lbm_layout ml) {
 %(eq_stores)s
 }
 
+// pylbm's split routines, generated from the same spec: transport
+// (pylbm/algorithm/base.py:273-296: fnew[k, x] = f[k, x - v_k] on the interior, a
+// copy of the stored words), relaxation (404-437: m_i <- (1 - s_i) m_i + s_i eq_i
+// on the whole halo-inclusive domain, FP64 moments) and source_term (439-504 with
+// algorithm/ode.py:11-16: one explicit Euler half step m_c <- m_c + dt/2 S(m) on
+// the interior, FP64 moments)
+template <typename S>
+__global__ void __launch_bounds__(256)
+transport_kernel(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Codec<S>::C> a) {
+  using B = typename Bits<S>::T;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nin[2]) return;
+  const int64_t cell = ((int64_t)(blockIdx.z + a.halo[0]) * a.n1 + blockIdx.y + a.halo[1]) * a.rs + a.lead + i +
+                       a.halo[2];
+  const B* fb = reinterpret_cast<const B*>(f);
+  B* gb = reinterpret_cast<B*>(fn);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) gb[a.dst_off[k] + cell] = fb[a.src_off[k] + cell];
+}
+
+__global__ void __launch_bounds__(256)
+relaxation_kernel(double* __restrict__ mo, lbm_layout ml, const StepArgs<double> a) {
+  using C = double;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= ml.n[2]) return;
+  const int64_t m_ = blockIdx.y, o_ = blockIdx.z;
+  const int64_t cm = cell_offset(ml, o_, m_, i);
+  const C* one_minus = a.one_minus;
+  const C* omega = a.omega;
+  (void)one_minus; (void)omega;
+%(rx_loads)s
+%(rx_body)s
+%(rx_stores)s
+}
+
+__global__ void __launch_bounds__(256)
+source_term_kernel(double* __restrict__ mo, lbm_layout ml, const StepArgs<double> a) {
+  using C = double;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nin[2]) return;
+  const int o = (int)blockIdx.z + a.halo[0], m = (int)blockIdx.y + a.halo[1], ii = i + a.halo[2];
+  (void)o; (void)m;
+  constexpr int j = 0;
+  (void)j;
+  const int64_t cm = cell_offset(ml, o, m, ii);
+  const C half_dt = a.half_dt;
+  const C tt = a.t;
+  (void)half_dt; (void)tt;
+%(st_coords)s
+%(st_loads)s
+%(st_body)s
+%(st_stores)s
+}
+
 // A value the compiler must treat as opaque: an equilibrium moment used as if it
 // had been stored and loaded again, so that none of its products is contracted
 // into an FMA of m2f's sums (which would round differently from the two-kernel path).
@@ -741,6 +795,18 @@ static int m2f_launch(const double* mi, const lbm_layout* ml, void* f, const lbm
 }
 
 template <typename S>
+static int transport_launch(const void* f, void* fn, const lbm_layout* fl, const lbm_params* p, hipStream_t s) {
+  StepArgs<typename Codec<S>::C> a;
+  int rc = make_args<typename Codec<S>::C>(fl, p, &a);
+  if (rc) return rc;
+  const unsigned bx = block_x(a.nin[2]);
+  dim3 grid((unsigned)((a.nin[2] + bx - 1) / bx), (unsigned)a.nin[1], (unsigned)a.nin[0]);
+  transport_kernel<S><<<grid, bx, 0, s>>>((const S*)f, (S*)fn, a);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename S>
 static int eq_m2f_launch(const double* mi, const lbm_layout* ml, void* f, const lbm_layout* fl,
                          const lbm_params* p, hipStream_t s) {
   StepArgs<double> a;
@@ -811,6 +877,39 @@ int lbm_gen_equilibrium(double* d_m, const lbm_layout* ml, void* stream) {
   const unsigned bx = block_x(ml->n[2]);
   dim3 grid((unsigned)((ml->n[2] + bx - 1) / bx), (unsigned)ml->n[1], (unsigned)ml->n[0]);
   equilibrium_kernel<<<grid, bx, 0, (hipStream_t)stream>>>(d_m, *ml);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int lbm_gen_transport(const void* d_f, void* d_fnew, const lbm_layout* fl, const lbm_params* params,
+                      void* stream) {
+  LBM_REQUIRE(d_f != d_fnew, "transport needs distinct source and destination arrays");
+  hipStream_t s = (hipStream_t)stream;
+  LBM_DISPATCH_STORAGE(fl ? fl->storage : -1, transport_launch, d_f, d_fnew, fl, params, s);
+}
+
+int lbm_gen_relaxation(double* d_m, const lbm_layout* ml, const lbm_params* params, void* stream) {
+  LBM_REQUIRE(params != nullptr, "NULL params");
+  StepArgs<double> a;
+  int rc = make_args<double>(ml, params, &a);
+  if (rc) return rc;
+  LBM_REQUIRE(ml->storage == LBM_F64, "relaxation needs FP64 moments");
+  const unsigned bx = block_x(ml->n[2]);
+  dim3 grid((unsigned)((ml->n[2] + bx - 1) / bx), (unsigned)ml->n[1], (unsigned)ml->n[0]);
+  relaxation_kernel<<<grid, bx, 0, (hipStream_t)stream>>>(d_m, *ml, a);
+  LBM_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int lbm_gen_source_term(double* d_m, const lbm_layout* ml, const lbm_params* params, void* stream) {
+  LBM_REQUIRE(params != nullptr, "NULL params");
+  StepArgs<double> a;
+  int rc = make_args<double>(ml, params, &a);
+  if (rc) return rc;
+  LBM_REQUIRE(ml->storage == LBM_F64, "source_term needs FP64 moments");
+  const unsigned bx = block_x(a.nin[2]);
+  dim3 grid((unsigned)((a.nin[2] + bx - 1) / bx), (unsigned)a.nin[1], (unsigned)a.nin[0]);
+  source_term_kernel<<<grid, bx, 0, (hipStream_t)stream>>>(d_m, *ml, a);
   LBM_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -1203,6 +1302,16 @@ def generate_source(spec: KernelSpec) -> str:
                    + ["C m%d;" % i for i in range(spec.nconsm, ns)])
     eqm2f_opaque = ["m%d = stored_moment(m%d);" % (i, i) for i in range(spec.nconsm, ns)]
 
+    # split routines: relaxation (every relaxed row) and source_term (the source rows)
+    relaxed_rows = [i for i in range(spec.nconsm, ns) if spec.s[i] != 0]
+    rx_loads = ["C m%d = mo[%d * ml.pop_stride + cm];" % (i, i) for i in range(ns)]
+    rx_loads += ["(void)m%d;" % i for i in range(ns)]
+    rx_body = _collision_lines(spec, with_sources=False)
+    rx_stores = ["mo[%d * ml.pop_stride + cm] = m%d;" % (i, i) for i in relaxed_rows]
+    st_loads = list(rx_loads)
+    st_body = ["m%d = m%d + half_dt * (%s);" % (idx, idx, _ccode(sp.sympify(expr))) for idx, expr in spec.sources]
+    st_stores = ["mo[%d * ml.pop_stride + cm] = m%d;" % (idx, idx) for idx in sorted({i for i, _ in spec.sources})]
+
     # cell-centre coordinates of the reference axes the source terms use (pylbm's X, Y, Z)
     used = set()
     for _, e in spec.sources:
@@ -1273,5 +1382,8 @@ def generate_source(spec: KernelSpec) -> str:
         f2m_loads=_indent(f2m_loads), f2m_full=_indent(f2m_full), f2m_stores=_indent(f2m_stores),
         m2f_loads=_indent(m2f_loads), m2f_full=_indent(m2f_full), m2f_stores=_indent(m2f_stores),
         eq_loads=_indent(eq_loads), eq_body=_indent(eq_body), eq_stores=_indent(eq_stores),
+        rx_loads=_indent(rx_loads), rx_body=_indent(rx_body), rx_stores=_indent(rx_stores),
+        st_coords=_indent(coords), st_loads=_indent(st_loads), st_body=_indent(st_body),
+        st_stores=_indent(st_stores),
         eqm2f_loads=_indent(eqm2f_loads), eqm2f_opaque=_indent(eqm2f_opaque),
     )

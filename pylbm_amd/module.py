@@ -10,6 +10,9 @@ argument names, over the C ABI of include/pylbm_hip.h:
 
 * ``one_time_step(f, fnew, t, dt)``       generated one_time_step (pull.py:11-59)
 * ``f2m(f, m)``, ``m2f(m, f)``, ``equilibrium(m)``   generated routines (base.py:298-402)
+* ``transport(f, fnew)``, ``relaxation(m)``, ``source_term(m, t, dt)``
+                                          generated split routines (base.py:273-296,
+                                          404-437, 439-504)
 * ``update(f)``                           Array.update: periodic wrap of every
                                           halo slot (storage.py:306-367)
 * ``bounce_back``, ``anti_bounce_back``   (f, istore, iload0, rhs, ncond)
@@ -93,6 +96,20 @@ class HipModule:
     def m2f(self, m, f):
         self._ok(self.gen.lbm_gen_m2f(self._p(m), ctypes.byref(self.mlayout), self._p(f),
                                       ctypes.byref(self.layout), ctypes.byref(self.params), self.stream), "m2f")
+
+    def transport(self, f, fnew):
+        self._ok(self.gen.lbm_gen_transport(self._p(f), self._p(fnew), ctypes.byref(self.layout),
+                                            ctypes.byref(self.params), self.stream), "transport")
+
+    def relaxation(self, m):
+        self._ok(self.gen.lbm_gen_relaxation(self._p(m), ctypes.byref(self.mlayout), ctypes.byref(self.params),
+                                             self.stream), "relaxation")
+
+    def source_term(self, m, t, dt):
+        self.params.t = float(t)
+        self.params.dt = float(dt)
+        self._ok(self.gen.lbm_gen_source_term(self._p(m), ctypes.byref(self.mlayout), ctypes.byref(self.params),
+                                              self.stream), "source_term")
 
     def equilibrium(self, m):
         self._ok(self.gen.lbm_gen_equilibrium(self._p(m), ctypes.byref(self.mlayout), self.stream), "equilibrium")

@@ -147,6 +147,9 @@ GEN_SIGNATURES = {
     "lbm_gen_equilibrium": ([_P, ctypes.POINTER(LbmLayout), _P], _I),
     "lbm_gen_eq_m2f": ([_P, ctypes.POINTER(LbmLayout), _P, ctypes.POINTER(LbmLayout),
                         ctypes.POINTER(LbmParams), _P], _I),
+    "lbm_gen_transport": ([_P, _P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmParams), _P], _I),
+    "lbm_gen_relaxation": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmParams), _P], _I),
+    "lbm_gen_source_term": ([_P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmParams), _P], _I),
     "lbm_gen_run_blocked": ([_P, _P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmParams), _I64, _I, _I, _I,
                              ctypes.POINTER(_I), _P], _I),
     "lbm_gen_run": ([_P, _P, ctypes.POINTER(LbmLayout), ctypes.POINTER(LbmVelocities),

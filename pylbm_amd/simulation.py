@@ -81,13 +81,14 @@ class _Container:
 
     gpu_support = True
 
-    def __init__(self, rt, ns, shape_halo, vmax, storage, consm, wshift, interleaved=True):
+    def __init__(self, rt, ns, shape_halo, vmax, storage, consm, wshift, interleaved=True, vel=None):
         self.F = DeviceArray(rt, ns, shape_halo, vmax, storage, wshift, interleaved)
         self.Fnew = DeviceArray(rt, ns, shape_halo, vmax, storage, wshift, interleaved)
         self._m = None
-        self._rt, self._shape, self._vmax, self._consm = rt, shape_halo, vmax, consm
+        self._rt, self._shape, self._vmax, self._consm, self._vel = rt, shape_halo, vmax, consm, vel
         for a in (self.F, self.Fnew):
             a.set_conserved_moments(consm)
+            a.vel = vel
         self.nv = ns
         self.sorder = list(range(len(shape_halo) + 1))
 
@@ -96,6 +97,7 @@ class _Container:
         if self._m is None:
             self._m = DeviceArray(self._rt, self.nv, self._shape, self._vmax, LBM_F64)
             self._m.set_conserved_moments(self._consm)
+            self._m.vel = self._vel
         return self._m
 
 
@@ -160,7 +162,8 @@ class Simulation:
         if layout not in ("interleaved", "planes"):
             raise ValueError("layout must be 'interleaved' or 'planes'")
         self.container = _Container(self.rt, self.ns, self.domain.shape_halo, vmax, self.storage,
-                                    self.scheme.consm, wshift, interleaved=(layout == "interleaved"))
+                                    self.scheme.consm, wshift, interleaved=(layout == "interleaved"),
+                                    vel=self.vel)
         self.bc = Boundary(self.domain, dico)
         for method in self.bc.methods:
             method.set_iload()
@@ -239,6 +242,32 @@ class Simulation:
 
     def f2m(self):
         self._f2m(self.container.F, self.container.m)
+
+    def transport(self):
+        """pylbm's transport phase (pylbm/simulation.py:322-327, the generated routine of
+        pylbm/algorithm/base.py:273-296): Fnew[k, x] = F[k, x - v_k] on the interior.
+        F's ghost slots are read as they are, and F and Fnew are not exchanged."""
+        c = self.container
+        self._check(self.module.lbm_gen_transport(ctypes.c_void_p(c.F.ptr), ctypes.c_void_p(c.Fnew.ptr),
+                                                  ctypes.byref(c.F.layout), ctypes.byref(self.params),
+                                                  self.rt.stream), "transport")
+
+    def relaxation(self):
+        """pylbm's relaxation phase on the moment array m (pylbm/simulation.py:329-334,
+        pylbm/algorithm/base.py:404-437): m_i <- (1 - s_i) m_i + s_i eq_i on every cell."""
+        m = self.container.m
+        self._check(self.module.lbm_gen_relaxation(ctypes.c_void_p(m.ptr), ctypes.byref(m.layout),
+                                                   ctypes.byref(self.params), self.rt.stream), "relaxation")
+
+    def source_term(self, fraction_of_time_step=1.0):
+        """pylbm's source-term phase on the moment array m (pylbm/simulation.py:336-342,
+        pylbm/algorithm/base.py:439-504 with ode.py:11-16): one explicit Euler half step
+        m_c <- m_c + dt/2 S(m, X, Y, Z, t) on the interior at the current t.  As in
+        pylbm, fraction_of_time_step is accepted and does not change the step."""
+        m = self.container.m
+        self.params.t = self.t
+        self._check(self.module.lbm_gen_source_term(ctypes.c_void_p(m.ptr), ctypes.byref(m.layout),
+                                                    ctypes.byref(self.params), self.rt.stream), "source_term")
 
     # ---------------------------------------------------------- init
     def _initialize(self):
@@ -680,9 +709,19 @@ class Simulation:
         self._moments()
         return self.container.m._in(i)
 
+    @m_halo.setter
+    def m_halo(self, i, value):
+        self._update_m = False
+        self.container.m[i] = value
+
     @itemproperty
     def F_halo(self, i):
         return self.container.F[i]
+
+    @F_halo.setter
+    def F_halo(self, i, value):
+        self._update_m = True
+        self.container.F[i] = value
 
     @itemproperty
     def F(self, i):

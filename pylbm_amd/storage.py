@@ -9,6 +9,8 @@ shifted so the first interior element of each row is 128-byte aligned.  Host
 copies are made only on access (``arr[k]``, ``arr._in(k)``), like the
 Loo.py arrays' ``.get()`` (pylbm/storage.py:126-157).
 """
+import ctypes
+
 import numpy as np
 import sympy as sp
 
@@ -89,6 +91,7 @@ class DeviceArray:
         self.ptr = rt.malloc(self.nbytes)
         self.consm = {}
         self.wshift = np.zeros(ns) if wshift is None else np.asarray(wshift, dtype=np.float64)
+        self.vel = None   # the scheme's lbm_velocities, given by the container (update)
 
     def __del__(self):
         try:
@@ -97,6 +100,14 @@ class DeviceArray:
                 self.ptr = None
         except Exception:
             pass
+
+    def update(self):
+        """Array.update (pylbm/storage.py:306-367): every ghost slot of every field takes
+        the periodic wrap of the interior, axis by axis (asynchronous on the stream)."""
+        if self.vel is None:
+            raise RuntimeError("update needs the velocities of the array's scheme")
+        self.rt.check(self.rt.lib.lbm_ghost_fill_all(ctypes.c_void_p(self.ptr), ctypes.byref(self.layout),
+                                                     ctypes.byref(self.vel), self.rt.stream), "update")
 
     # ------------------------------------------------------------ geometry
     @property

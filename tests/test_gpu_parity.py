@@ -700,3 +700,47 @@ def test_temporal_blocking_bit_identical(hip, name, dtype):
     for k in range(a.ns):
         assert np.isfinite(b.F[k]).all(), (name, dtype, k)
         assert np.array_equal(a.F[k], b.F[k]), (name, dtype, k, shape)
+
+
+# pylbm's split routines (Simulation.transport / relaxation / source_term and
+# Array.update, pylbm/simulation.py:322-342, pylbm/storage.py:306-367) on the device,
+# fed the oracle's own state so that the comparison is of the routine alone
+@pytest.mark.parametrize("name", ["forced", "lid", "taylor_green", "relvel"])
+def test_split_routines_match_oracle(hip, name):
+    d = {"forced": lambda: cases.forced_periodic_d2q9(),
+         "lid": lambda: cases.lid_cavity_d2q9(16),
+         "taylor_green": lambda: cases.taylor_green_d3q19(8),
+         "relvel": lambda: __import__("pylbm_amd.demos", fromlist=["x"]).lid_driven_cavity(1.0 / 16)[0]}[name]()
+    sim, ref = Simulation(d), Oracle(d)
+    for _ in range(3):
+        ref.one_time_step()
+    sim.t, sim.nt = ref.t, ref.nt
+    c = sim.container
+    # update: the periodic wrap of every ghost slot, the same copies as the oracle's
+    f = ref.F.copy()
+    ref.ghost_fill(f)
+    c.F.set_fields(ref.F)
+    c.F.update()
+    assert np.array_equal(c.F.fields(), f)
+    # transport: a copy, bit for bit on the interior
+    c.F.set_fields(f)
+    sim.transport()
+    want = np.zeros_like(f)
+    ref.transport(f, want)
+    inner = ref._interior()
+    assert np.array_equal(c.Fnew.fields()[inner], want[inner])
+    # relaxation on every cell and source_term on the interior, FP64 moments
+    m = ref._f2m(ref.F)
+    c.m.set_fields(m)
+    sim.relaxation()
+    ref.relaxation(m)
+    got = c.m.fields()
+    assert rel_err(got, m, floor=1e-3) <= TOL64
+    relaxed = m.copy()
+    c.m.set_fields(m)
+    sim.source_term()
+    ref.source_term(m)
+    got = c.m.fields()
+    assert rel_err(got, m, floor=1e-3) <= TOL64
+    # the forced box has sources in X, Y and t: the routine changed the moments
+    assert np.array_equal(got, relaxed) == (name != "forced")

@@ -287,3 +287,73 @@ def test_adam_first_step_is_lr_sign():
     p = [[np.zeros(4)]]
     Adam(p, lr=1e-2, eps=1e-12).step(p, [[g]])
     assert np.allclose(p[0][0], -1e-2 * np.sign(g), rtol=1e-9, atol=0)
+
+
+# -------------------------------------------------------- split routines
+
+def test_split_transport_is_periodic_shift():
+    """transport on a state whose ghost slots hold the periodic wrap is, per population,
+    the periodic shift of the interior by v_k (np.roll); a sign error in the pull
+    (x + v_k) or a shifted interior window fails it."""
+    d = cases.taylor_green_d3q19(6, generator="numpy")
+    sim = Simulation(d)
+    rng = np.random.default_rng(3)
+    sim.F[...] = rng.standard_normal(sim.F.shape)
+    sim.ghost_fill(sim.F)
+    sim.Fnew[...] = 0.0
+    sim.transport()
+    inner = sim.F[sim._interior()]
+    got = sim.Fnew[sim._interior()]
+    for k in range(sim.ns):
+        want = np.roll(inner[k], shift=tuple(int(c) for c in sim.v[k]), axis=tuple(range(sim.dim)))
+        assert np.array_equal(got[k], want), k
+
+
+def test_split_relaxation_contracts_towards_equilibrium():
+    """relaxation: every relaxed row's distance to the equilibrium shrinks by exactly
+    (1 - s_i) (the equilibrium depends only on the conserved moments, which are not
+    relaxed), the conserved rows are untouched, and the equilibrium state is a fixed
+    point; s_i and 1 - s_i swapped, or eq taken after the update, fail it."""
+    d = cases.lid_cavity_d2q9(8, generator="numpy")
+    sim = Simulation(d)
+    rng = np.random.default_rng(4)
+    m = sim._f2m(sim.F) + 1e-3 * rng.standard_normal(sim.F.shape)
+    m0 = m.copy()
+    eq0 = sim._eq_full(m0)
+    sim.relaxation(m)
+    nc = sim.scheme.nconsm
+    assert np.array_equal(m[:nc], m0[:nc])
+    for i in range(nc, sim.ns):
+        s = sim.scheme.s[i]
+        np.testing.assert_allclose(m[i] - eq0[i], (1 - s) * (m0[i] - eq0[i]), rtol=1e-12, atol=1e-15)
+    fixed = eq0.copy()
+    sim.relaxation(fixed)
+    np.testing.assert_allclose(fixed, eq0, rtol=0, atol=1e-15)
+
+
+def test_split_source_term_half_step():
+    """source_term: one explicit Euler half step at the current t on the interior only:
+    for S(qx) = rho g0 (1 + Y)(1 + t), S(qy) = rho g0 X / 4 the momentum rows grow by
+    dt/2 times those values at the cell centres, rho and the ghost cells are unchanged;
+    a full step, swapped coordinates or t + dt fail it."""
+    g0 = 1e-3
+    d = cases.forced_periodic_d2q9(g0=g0, generator="numpy")
+    sim = Simulation(d)
+    sim.one_time_step()
+    sim.one_time_step()
+    m = sim._f2m(sim.F)
+    m0 = m.copy()
+    sim.source_term(m)
+    X, Y = np.meshgrid(*sim.domain.coords, indexing="ij")
+    c = sim.scheme.consm
+    irho, iqx, iqy = c[cases.RHO], c[cases.QX], c[cases.QY]
+    inner = sim._interior()[1:]
+    rho = m0[irho][inner]
+    np.testing.assert_allclose(m[iqx][inner] - m0[iqx][inner], 0.5 * sim.dt * rho * g0 * (1 + Y) * (1 + sim.t),
+                               rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(m[iqy][inner] - m0[iqy][inner], 0.5 * sim.dt * rho * g0 * X / 4,
+                               rtol=1e-12, atol=1e-18)
+    assert np.array_equal(m[irho], m0[irho])
+    ghost = np.ones(m.shape[1:], dtype=bool)
+    ghost[inner] = False
+    assert np.array_equal(m[:, ghost], m0[:, ghost])
