@@ -10,7 +10,8 @@ ABI of include/pylbm_hip.h.  Names follow pylbm: ``Simulation``, ``Domain``,
 """
 from . import bc  # noqa: F401
 from .domain import Domain  # noqa: F401
-from .elements import Circle, Parallelogram, Sphere, Triangle  # noqa: F401
+from .elements import (Circle, CylinderCircle, CylinderEllipse, CylinderTriangle, Ellipse,  # noqa: F401
+                       Ellipsoid, Parallelepiped, Parallelogram, Sphere, Triangle)
 from .hdf5 import H5File  # noqa: F401
 from .monitoring import Monitor  # noqa: F401
 from .scheme import Scheme  # noqa: F401

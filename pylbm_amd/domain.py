@@ -14,7 +14,8 @@ properties for small domains.
 """
 import numpy as np
 
-from .elements import Circle, Parallelogram, Sphere, Triangle  # noqa: F401  (re-export)
+from .elements import (Circle, CylinderCircle, CylinderEllipse, CylinderTriangle, Ellipse,  # noqa: F401
+                       Ellipsoid, Parallelepiped, Parallelogram, Sphere, Triangle)  # re-export
 from .stencil import Stencil, num_from_velocity
 
 __all__ = ["Domain"]
