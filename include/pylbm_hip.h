@@ -260,6 +260,17 @@ typedef struct lbm_params {
    * temporally blocked run: bits 0-7 the source population, bits 8-15 and 16-23
    * the mid / inner offsets + 128 (within one cell; 2D schemes) */
   const uint32_t* bbtab_rel;
+  /* NULL, or per table entry a second source for the interpolated (Bouzidi)
+   * links of pylbm/boundary.py:472-760: where bbtab_src2[e] is not 0xffffffff the
+   * entry is sign[e] * f[src[e]] + c2[e] * f[src2[e]] + rhs[e] (the arithmetic of
+   * lbm_links_apply_bouzidi, ca = sign, cb = c2, each product and sum rounded
+   * on its own).  bbtab_rel2: the relative form of src2 (as bbtab_rel) with bit
+   * 31 set on the entries that have a second source; the others repeat their
+   * bbtab_rel word.  The temporally blocked run needs bbtab_rel2 when bbtab_src2
+   * is set. */
+  const uint32_t* bbtab_src2;
+  const void* bbtab_c2;
+  const uint32_t* bbtab_rel2;
   /* cell-centre coordinates for source terms that depend on space (pylbm's X, Y,
    * Z): reference axis d of the cell at halo-inclusive index i is at
    * origin[d] + dx * i (the coordinates of pylbm's Domain.coords_halo) */

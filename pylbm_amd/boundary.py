@@ -207,42 +207,75 @@ MAX_LINK_GROUP = 8  # lists per launch of lbm_links_apply_sets
 def resolve_step(lists):
     """The step's link lists, in order, as one value per stored slot.
 
-    lists: [(store, load, sign, rhs or None)] (flat indices; rhs None: the list
-    adds nothing).  Each list reads the values the lists before it left (its loads
-    are looked up among their stores) and the last store of a slot wins.  Returns
-    (slot, origin, sign, rhs, has_rhs, two_rhs), sorted by slot: the final value of
-    slot is sign * f[origin] + rhs (rhs only if has_rhs) with origin a slot no list
-    stores; two_rhs marks chains that applied two rhs (their value needs two
-    roundings, which sign * f + rhs does not reproduce)."""
+    lists: [(store, load, sign, rhs or None)] or, for an interpolated (Bouzidi)
+    list, [(store, load0, ca, rhs or None, load1, cb)] (flat indices; rhs None:
+    the list adds nothing).  Each list reads the values the lists before it left
+    (its loads are looked up among their stores) and the last store of a slot
+    wins.  Returns (slot, origin, sign, rhs, has_rhs, two_rhs, origin2, c2), sorted
+    by slot: the final value of slot is sign * f[origin] (+ c2 * f[origin2] where
+    origin2 >= 0) + rhs (rhs only if has_rhs) with origins slots no list stores;
+    two_rhs marks values that this form does not reproduce bit for bit: chains that
+    applied two rhs (their value needs two roundings), interpolations of an
+    interpolated value or of a value that carries an rhs, and an rhs added to an
+    interpolated value (a signed copy of one is exact)."""
     slot = np.zeros(0, np.int64)
     org = np.zeros(0, np.int64)
     sg = np.zeros(0)
     rh = np.zeros(0)
     has = np.zeros(0, bool)
     bad = np.zeros(0, bool)
-    for st, ld, s1, r1 in lists:
+    org2 = np.zeros(0, np.int64)
+    cc2 = np.zeros(0)
+
+    def lookup(ld):
+        if not slot.size:
+            return np.zeros(ld.size, bool), np.zeros(0, np.int64)
+        pos = np.minimum(np.searchsorted(slot, ld), slot.size - 1)
+        hit = slot[pos] == ld
+        return hit, pos[hit]
+
+    for entry in lists:
+        st, ld, s1, r1 = entry[:4]
         st = np.asarray(st, np.int64)
         ld = np.asarray(ld, np.int64)
         s1 = np.asarray(s1, np.float64)
         h1 = np.full(st.size, r1 is not None)
         r1 = np.asarray(r1, np.float64) if r1 is not None else np.zeros(st.size)
         o2, s2, r2, h2, b2 = ld.copy(), s1.copy(), r1.copy(), h1.copy(), np.zeros(st.size, bool)
-        if slot.size:
-            pos = np.minimum(np.searchsorted(slot, ld), slot.size - 1)
-            hit = slot[pos] == ld
-            p = pos[hit]
+        q2, c2 = np.full(st.size, -1, np.int64), np.zeros(st.size)
+        interp = len(entry) > 4 and entry[4] is not None
+        hit, p = lookup(ld)
+        if interp:
+            # ca f[load0] + cb f[load1] + rhs: a load may take a signed copy (exact)
+            q2 = np.asarray(entry[4], np.int64).copy()
+            c2 = np.asarray(entry[5], np.float64).copy()
+            o2[hit] = org[p]
+            s2[hit] = s1[hit] * sg[p]
+            b2[hit] = has[p] | bad[p] | (org2[p] >= 0)
+            hit1, p1 = lookup(q2)
+            q2[hit1] = org[p1]
+            c2[hit1] = c2[hit1] * sg[p1]
+            b2[hit1] |= has[p1] | bad[p1] | (org2[p1] >= 0)
+        elif p.size:
             o2[hit] = org[p]
             s2[hit] = s1[hit] * sg[p]
             # s1 (sg f + rh) + r1, exact when at most one of rh, r1 applies
             r2[hit] = np.where(has[p], s1[hit] * rh[p], 0.0) + np.where(h1[hit], r1[hit], 0.0)
             h2[hit] = h1[hit] | has[p]
-            b2[hit] = bad[p] | (h1[hit] & has[p])
+            # a signed copy of an interpolated value is the interpolation with every
+            # coefficient and the rhs signed (negation is exact); one with an rhs of
+            # its own would round twice
+            ih = org2[p] >= 0
+            q2[hit] = np.where(ih, org2[p], -1)
+            c2[hit] = np.where(ih, s1[hit] * cc2[p], 0.0)
+            b2[hit] = bad[p] | (h1[hit] & has[p]) | (ih & h1[hit])
         # the stores of this list replace earlier stores of the same slots
-        cat = [np.concatenate(a) for a in ((slot, st), (org, o2), (sg, s2), (rh, r2), (has, h2), (bad, b2))]
+        cat = [np.concatenate(a) for a in ((slot, st), (org, o2), (sg, s2), (rh, r2), (has, h2), (bad, b2),
+                                           (org2, q2), (cc2, c2))]
         uniq, first = np.unique(cat[0][::-1], return_index=True)
         keep = cat[0].size - 1 - first
-        slot, org, sg, rh, has, bad = uniq, cat[1][keep], cat[2][keep], cat[3][keep], cat[4][keep], cat[5][keep]
-    return slot, org, sg, rh, has, bad
+        slot, org, sg, rh, has, bad, org2, cc2 = (uniq,) + tuple(c[keep] for c in cat[1:])
+    return slot, org, sg, rh, has, bad, org2, cc2
 
 
 def merge_flags(lists):

@@ -218,8 +218,14 @@ __global__ void __launch_bounds__(256) bouzidi_kernel(S* __restrict__ f,
   if (t >= n) return;
   const C v0 = Codec<S>::load_dev(f, load0[t], (C)0);
   const C v1 = Codec<S>::load_dev(f, load1[t], (C)0);
-  C r = ca[t] * v0 + cb[t] * v1;
-  if (rhs) r += rhs[t];
+  C r;
+  {
+    // every product and sum rounds on its own (no FMA): the arithmetic the generated
+    // step uses for the same links folded into its pulls, and numpy's
+#pragma clang fp contract(off)
+    r = ca[t] * v0 + cb[t] * v1;
+    if (rhs) r += rhs[t];
+  }
   if (work)
     work[t] = r;
   else
@@ -324,16 +330,21 @@ __global__ void __launch_bounds__(256) links_group_kernel(S* __restrict__ f,
   while (t >= G.end[j]) ++j;
   const int64_t i = t - (j ? G.end[j - 1] : 0);
   C r;
-  if (G.load1[j]) {
-    const C v0 = Codec<S>::load_dev(f, G.load[j][i], (C)0);
-    const C v1 = Codec<S>::load_dev(f, G.load1[j][i], (C)0);
-    r = G.ca[j][i] * v0 + G.cb[j][i] * v1;
-  } else {
-    C v = Codec<S>::load_dev(f, G.load[j][i], (C)0);
-    const C a = G.sgn[j] ? (C)G.sgn[j][i] : (C)G.sign[j];
-    r = a * v;
+  {
+    // no FMA, as in bouzidi_kernel: the step's folded pulls round every product
+    // and sum on its own
+#pragma clang fp contract(off)
+    if (G.load1[j]) {
+      const C v0 = Codec<S>::load_dev(f, G.load[j][i], (C)0);
+      const C v1 = Codec<S>::load_dev(f, G.load1[j][i], (C)0);
+      r = G.ca[j][i] * v0 + G.cb[j][i] * v1;
+    } else {
+      C v = Codec<S>::load_dev(f, G.load[j][i], (C)0);
+      const C a = G.sgn[j] ? (C)G.sgn[j][i] : (C)G.sign[j];
+      r = a * v;
+    }
+    if (G.rhs[j]) r += G.rhs[j][i];
   }
-  if (G.rhs[j]) r += G.rhs[j][i];
   Codec<S>::store(f, G.store[j][i], r, (C)0);
 }
 

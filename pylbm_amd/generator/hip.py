@@ -445,6 +445,9 @@ struct StepArgs {
   const C* bbtab_sign;
   const C* bbtab_rhs;
   const uint32_t* bbtab_rel;       // NULL or the table sources relative to the pulling cell
+  const uint32_t* bbtab_src2;      // NULL or second sources (0xffffffff: a one-source entry)
+  const C* bbtab_c2;
+  const uint32_t* bbtab_rel2;
   C omega[NS], one_minus[NS], wshift[NS];
   C cref[NCREF];
   C half_dt, t;
@@ -711,6 +714,9 @@ static int make_args(const lbm_layout* L, const lbm_params* p, StepArgs<C>* a) {
   a->bbtab_sign = p ? (const C*)p->bbtab_sign : nullptr;
   a->bbtab_rhs = p ? (const C*)p->bbtab_rhs : nullptr;
   a->bbtab_rel = p ? p->bbtab_rel : nullptr;
+  a->bbtab_src2 = p ? p->bbtab_src2 : nullptr;
+  a->bbtab_c2 = p ? (const C*)p->bbtab_c2 : nullptr;
+  a->bbtab_rel2 = p ? p->bbtab_rel2 : nullptr;
   LBM_REQUIRE(a->bbmask == nullptr || BBSUB_OK, "this stencil has no symmetric pairs: no fused bounce-back");
   return 0;
 }
@@ -925,6 +931,8 @@ int lbm_gen_run_blocked(void* d_f, void* d_fnew, const lbm_layout* layout, lbm_p
   LBM_REQUIRE(tile_m >= 1 && tile_i >= 1, "tile sizes must be positive");
   LBM_REQUIRE(params->bbtab_bits == nullptr || params->bbtab_rel != nullptr,
               "folded links with table entries need their relative sources (bbtab_rel)");
+  LBM_REQUIRE(params->bbtab_src2 == nullptr || params->bbtab_rel2 != nullptr,
+              "folded interpolated links need their relative second sources (bbtab_rel2)");
   hipStream_t s = (hipStream_t)stream;
   void* a = d_f;
   void* b = d_fnew;
@@ -1206,6 +1214,8 @@ def generate_source(spec: KernelSpec) -> str:
                           "            using CC = typename Codec<S>::C;",
                           "            const uint32_t e_ = tbase + (uint32_t)__builtin_popcount(tb & %du);" % (bit - 1),
                           "            CC r_ = a.bbtab_sign[e_] * Codec<S>::decode_dev(f[a.bbtab_src[e_]], (CC)0);",
+                          "            const uint32_t s2_ = a.bbtab_src2 ? a.bbtab_src2[e_] : 0xffffffffu;",
+                          "            if (s2_ != 0xffffffffu) r_ = r_ + a.bbtab_c2[e_] * Codec<S>::decode_dev(f[s2_], (CC)0);",
                           "            r_ += a.bbtab_rhs[e_];",
                           "            L%d[j] = Codec<S>::encode(r_, (CC)0);" % k,
                           "          }",
@@ -1340,23 +1350,41 @@ def generate_source(spec: KernelSpec) -> str:
                                "      const S b_ = cur[%d * RC + lcell];" % ksym[k],
                                "      L%d[0] = (bits & %du) ? b_ : L%d[0];" % (k, 1 << k, k),
                                "    }"]
+            # Per entry: the one-source tables (no bbtab_rel2) read source, sign and
+            # value; tables with interpolated entries read the second relative
+            # source and coefficient with them, all five loads issued together (bit
+            # 31 of bbtab_rel2 flags the entries that have a second source; the
+            # others repeat their first), so no load waits on another.
             tb_sub += ["    if (bits & 0x80000000u) {",
                        "      using CC = typename Codec<S>::C;",
                        "      const uint32_t tb = tbits[lcell], tbase = tbase_l[lcell];"]
-            for k in range(ns):
-                if ksym[k] != k:
+            for two in (True, False):
+                tb_sub += ["      if (%sa.bbtab_rel2) {" % ("" if two else "!")]
+                for k in range(ns):
+                    if ksym[k] == k:
+                        continue
                     bit = 1 << k
-                    tb_sub += ["      if (tb & %du) {" % bit,
-                               "        const uint32_t e_ = tbase + (uint32_t)__builtin_popcount(tb & %du);" % (bit - 1),
-                               "        const uint32_t rel = a.bbtab_rel[e_];",
-                               "        const CC sg_ = a.bbtab_sign[e_];",
-                               "        const CC rh_ = a.bbtab_rhs[e_];",
-                               "        const int ks = (int)(rel & 0xffu), dm = (int)((rel >> 8) & 0xffu) - 128,",
-                               "                  di = (int)((rel >> 16) & 0xffu) - 128;",
-                               "        CC r_ = sg_ * Codec<S>::decode_dev(cur[ks * RC + lcell + dm * RI + di], (CC)0);",
-                               "        r_ += rh_;",
-                               "        L%d[0] = Codec<S>::encode(r_, (CC)0);" % k,
-                               "      }"]
+                    tb_sub += ["        if (tb & %du) {" % bit,
+                               "          const uint32_t e_ = tbase + (uint32_t)__builtin_popcount(tb & %du);" % (bit - 1),
+                               "          const uint32_t rel = a.bbtab_rel[e_];"]
+                    if two:
+                        tb_sub += ["          const uint32_t rel2 = a.bbtab_rel2[e_];",
+                                   "          const CC c2_ = a.bbtab_c2[e_];"]
+                    tb_sub += ["          const CC sg_ = a.bbtab_sign[e_];",
+                               "          const CC rh_ = a.bbtab_rhs[e_];",
+                               "          const int ks = (int)(rel & 0xffu), dm = (int)((rel >> 8) & 0xffu) - 128,",
+                               "                    di = (int)((rel >> 16) & 0xffu) - 128;",
+                               "          CC r_ = sg_ * Codec<S>::decode_dev(cur[ks * RC + lcell + dm * RI + di], (CC)0);"]
+                    if two:
+                        tb_sub += ["          if (rel2 & 0x80000000u) {",
+                                   "            const int ks2 = (int)(rel2 & 0xffu), dm2 = (int)((rel2 >> 8) & 0xffu) - 128,",
+                                   "                      di2 = (int)((rel2 >> 16) & 0xffu) - 128;",
+                                   "            r_ = r_ + c2_ * Codec<S>::decode_dev(cur[ks2 * RC + lcell + dm2 * RI + di2], (CC)0);",
+                                   "          }"]
+                    tb_sub += ["          r_ += rh_;",
+                               "          L%d[0] = Codec<S>::encode(r_, (CC)0);" % k,
+                               "        }"]
+                tb_sub += ["      }"]
             tb_sub += ["    }", "  }", "}"]
         tb_kernel = _TB_KERNEL % dict(
             tb_loads=_indent(tb_loads, 6), tb_sub=_indent(tb_sub, 6),

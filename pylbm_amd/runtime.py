@@ -52,6 +52,7 @@ class LbmParams(ctypes.Structure):
                 ("dt", ctypes.c_double), ("t", ctypes.c_double), ("bbmask", ctypes.c_void_p),
                 ("bbtab_bits", ctypes.c_void_p), ("bbtab_base", ctypes.c_void_p), ("bbtab_src", ctypes.c_void_p),
                 ("bbtab_sign", ctypes.c_void_p), ("bbtab_rhs", ctypes.c_void_p), ("bbtab_rel", ctypes.c_void_p),
+                ("bbtab_src2", ctypes.c_void_p), ("bbtab_c2", ctypes.c_void_p), ("bbtab_rel2", ctypes.c_void_p),
                 ("origin", ctypes.c_double * 3), ("dx", ctypes.c_double),
                 ("xcd_remap", ctypes.c_int32), ("vec_cells", ctypes.c_int32)]
 

@@ -26,7 +26,8 @@ import numpy as np
 import sympy as sp
 
 from . import runtime as rtm
-from .boundary import Boundary, DeviceLinks, host_lists, merge_flags, resolve_step, unwritten_ghost_reads
+from .boundary import (Boundary, DeviceLinks, bouzidi_lists, host_lists, merge_flags, resolve_step,
+                       unwritten_ghost_reads)
 from .boundary import _unflat as _unflat_fn
 from .domain import Domain
 from .generator.build import load_module
@@ -287,7 +288,8 @@ class Simulation:
                                                       self.wshift, flat=(st, ld)))
         for d in self._dev_links:
             d.fused = False
-        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_src", "bbtab_sign", "bbtab_rhs", "bbtab_rel"):
+        for name in ("bbmask", "bbtab_bits", "bbtab_base", "bbtab_src", "bbtab_sign", "bbtab_rhs", "bbtab_rel",
+                     "bbtab_src2", "bbtab_c2", "bbtab_rel2"):
             setattr(self.params, name, None)
         if self.fold_links:
             self._fold_links()
@@ -335,18 +337,28 @@ class Simulation:
 
     def _fold_step(self):
         """Every list of the step, resolved in order into one value per stored slot
-        (boundary.resolve_step): sign * f[origin] + rhs of a slot no list stores.
+        (boundary.resolve_step): sign * f[origin] + rhs of a slot no list stores, or
+        for an interpolated (Bouzidi) link ca * f[origin] + cb * f[origin2] + rhs.
         That reproduces the launches' result exactly when no chain applies two rhs
+        and no link reads an interpolated value or interpolates a value with an rhs
         (a copy or a sign flip does not round); otherwise, with a time-dependent
-        list, or when more than FOLD_TABLE_MAX links would need a table entry,
-        nothing is folded here."""
+        list, a two-phase interpolated list, or when more than FOLD_TABLE_MAX links
+        would need a table entry, nothing is folded here."""
         devs = self._dev_links
         if not devs or any(not isinstance(d.method, str) and d.method._tbc for d in devs):
             return False
-        if any(d.bouzidi for d in devs):
-            return False   # an interpolated value has two sources
         L, n, h, nin = self._fold_layout()
-        slot, org, sg, rh, has, bad = resolve_step([(d.h_store, d.h_load, d.h_sign, d.h_rhs) for d in devs])
+        lists = []
+        for d in devs:
+            if not d.bouzidi:
+                lists.append((d.h_store, d.h_load, d.h_sign, d.h_rhs))
+                continue
+            # an interpolated list: two sources per link, one phase only
+            st, l0, l1, ca, cb, rhs, two, index, shift = bouzidi_lists(d.method, L, self.wshift)
+            if two:
+                return False
+            lists.append((st, l0, ca, rhs if d.d_rhs is not None else None, l1, cb))
+        slot, org, sg, rh, has, bad, org2, c2 = resolve_step(lists)
         k, o, m, i = _unflat_fn(slot, L)
         vel = np.array([[self.vel.v[q][a] for a in range(3)] for q in range(self.ns)], dtype=np.int64)
         x = np.stack([o, m, i]) + vel[k].T
@@ -354,17 +366,19 @@ class Simulation:
         if np.any(bad & inside):
             return False
         k, x, org, sg, rh, has = k[inside], x[:, inside], org[inside], sg[inside], rh[inside], has[inside]
+        org2, c2 = org2[inside], c2[inside]
         xi = x - h[:, None]
         cell = (xi[0] * nin[1] + xi[1]) * nin[2] + xi[2]
         ksym = np.asarray(self.scheme.stencil.get_symmetric())[k]
         own = ksym * L.pop_stride + (x[0] * L.n[1] + x[1]) * L.row_stride + L.lead + x[2]
-        copy = (org == own) & (sg == 1.0) & ~has
+        copy = (org == own) & (sg == 1.0) & ~has & (org2 < 0)
         if np.count_nonzero(~copy) > FOLD_TABLE_MAX:
             return False
         mask = np.zeros(int(np.prod(nin)), dtype=np.uint32)
         np.bitwise_or.at(mask, cell, np.uint32(1) << k.astype(np.uint32))
         t = ~copy
-        self._upload_fold(mask, (cell[t], k[t], org[t], sg[t], np.where(has[t], rh[t], -0.0)), devs[0])
+        self._upload_fold(mask, (cell[t], k[t], org[t], sg[t], np.where(has[t], rh[t], -0.0), org2[t], c2[t]),
+                          devs[0])
         for d in devs:
             d.fused = True
         return True
@@ -374,7 +388,10 @@ class Simulation:
         f[k, x - v_k] = sign f[ksym, x] + value with x interior, no other folded link
         targets the same (k, x), and no later list of the step loads or stores the
         list's stored slots or stores its loaded slots (earlier lists run before
-        the pull either way)."""
+        the pull either way).  Interpolated (Bouzidi) lists the same way with a
+        second source: f[k, x - v_k] = ca f[ksym, x] + cb f[load1] + value, when the
+        list runs one-phase and no list folded before it stores a slot it loads
+        (those stores are never written)."""
         devs = self._dev_links
         layout = self.container.F.layout
         sten = self.scheme.stencil
@@ -385,8 +402,18 @@ class Simulation:
         mask = np.zeros(int(np.prod(shape_in)), dtype=np.uint32)
         tab = []
         owner = None
+        folded_stores = []
         for j, d in enumerate(devs):
             m = d.method
+            if (not isinstance(m, str) and getattr(m.kind, "bouzidi", False) and not m._tbc and not d.conflict
+                    and d.n > 0):
+                entry = self._fold_bouzidi(d, mask, devs[:j], devs[j + 1:])
+                if entry is not None:
+                    tab.append(entry)
+                    folded_stores.append(d.h_store)
+                    d.fused = True
+                    owner = owner or d
+                continue
             if (isinstance(m, str) or m.kind.name not in ("bounce_back", "anti_bounce_back") or m._tbc
                     or d.conflict or d.n == 0):
                 continue
@@ -409,15 +436,59 @@ class Simulation:
             if any(np.isin(o.h_store, d.h_store).any() or np.isin(o.h_load, d.h_store).any()
                    or np.isin(o.h_store, d.h_load).any() for o in devs[j + 1:]):
                 continue
+            if any(np.isin(ld, fs).any() for fs in folded_stores):
+                continue
             np.bitwise_or.at(mask, cell, bit)
             if d.h_rhs is not None or d.d_sign is not None or d.sign != 1.0:
                 # the link kernel adds rhs only when the list has one: -0.0 adds nothing
-                tab.append((cell, k, ld, sign, rhs if d.h_rhs is not None else np.full(d.n, -0.0)))
+                tab.append((cell, k, ld, sign, rhs if d.h_rhs is not None else np.full(d.n, -0.0),
+                            np.full(d.n, -1, dtype=np.int64), np.zeros(d.n)))
+            folded_stores.append(d.h_store)
             d.fused = True
             owner = owner or d
         if owner is not None:
-            cat = tuple(np.concatenate([t[q] for t in tab]) for q in range(5)) if tab else None
+            cat = tuple(np.concatenate([t[q] for t in tab]) for q in range(7)) if tab else None
             self._upload_fold(mask, cat, owner)
+
+    def _fold_bouzidi(self, d, mask, earlier, later):
+        """The table entries (cell, k, load0, ca, rhs, load1, cb) of one interpolated
+        list folded into the pulls, its mask bits set; None (mask untouched) when the
+        list does not fold: a link whose first load is not the pulling cell's own
+        ksym population or whose pulling cell is not interior, a direction already
+        folded at a cell, a later list touching its slots, or a load of a slot that
+        an earlier list stores (the ghost copy aside: it stores the periodic wrap the
+        folded loads read anyway)."""
+        layout = self.container.F.layout
+        st, l0, l1, ca, cb, rhs, two, index, shift = bouzidi_lists(d.method, layout, self.wshift)
+        if two or st.size == 0:
+            return None
+        L, n, h, nin = self._fold_layout()
+        k, o, m_, i = _unflat_fn(st, L)
+        vel = np.array([[self.vel.v[q][a] for a in range(3)] for q in range(self.ns)], dtype=np.int64)
+        x = np.stack([o, m_, i]) + vel[k].T
+        if np.any(x < h[:, None]) or np.any(x >= (n - h)[:, None]):
+            return None
+        ksym = np.asarray(self.scheme.stencil.get_symmetric())[k]
+        own = ksym * L.pop_stride + (x[0] * L.n[1] + x[1]) * L.row_stride + L.lead + x[2]
+        if not np.array_equal(l0, own):
+            return None
+        xi = x - h[:, None]
+        cell = (xi[0] * nin[1] + xi[1]) * nin[2] + xi[2]
+        bit = np.uint32(1) << k.astype(np.uint32)
+        if np.any(mask[cell] & bit) or np.unique(cell * 64 + k).size != cell.size:
+            return None
+        loads = np.concatenate([l0, l1])
+        if any(np.isin(o_.h_store, st).any() or np.isin(o_.h_load, st).any() or np.isin(o_.h_store, loads).any()
+               for o_ in later):
+            return None
+        if any(not isinstance(o_.method, str) and np.isin(o_.h_store, loads).any() for o_ in earlier):
+            return None
+        if np.isin(loads, st).any():
+            return None
+        np.bitwise_or.at(mask, cell, bit)
+        value = (np.asarray(d.method.rhs, dtype=np.float64)[index] + shift) if d.d_rhs is not None \
+            else np.full(st.size, -0.0)
+        return cell, k, l0, ca, value, l1, cb
 
     def _wrap_ghost_sources(self, src):
         """Table sources that are ghost slots, moved to the fully wrapped interior slot.
@@ -459,10 +530,13 @@ class Simulation:
         """Device copies of the fold mask and table (cell, k, src, sign, rhs) -> params."""
         up = []
         if tab is not None and tab[0].size:
-            cell, kk, src, sg, rh = tab
+            cell, kk, src, sg, rh, src2, c2 = tab
             src = self._wrap_ghost_sources(src)
+            two = src2 >= 0
+            if two.any():
+                src2 = np.where(two, self._wrap_ghost_sources(np.where(two, src2, src)), -1)
             o = np.lexsort((kk, cell))
-            cell, kk, src, sg, rh = cell[o], kk[o], src[o], sg[o], rh[o]
+            cell, kk, src, sg, rh, src2, c2, two = cell[o], kk[o], src[o], sg[o], rh[o], src2[o], c2[o], two[o]
             bits = np.zeros(mask.size, dtype=np.uint32)
             np.bitwise_or.at(bits, cell, np.uint32(1) << kk.astype(np.uint32))
             base = np.zeros(mask.size, dtype=np.uint32)
@@ -474,6 +548,17 @@ class Simulation:
             arrays = [("bbtab_bits", bits), ("bbtab_base", base), ("bbtab_src", src.astype(np.uint32)),
                       ("bbtab_sign", sg.astype(ctype)), ("bbtab_rhs", rh.astype(ctype))]
             rel = self._relative_sources(cell, src)
+            if two.any():
+                arrays += [("bbtab_src2", np.where(two, src2, 0xFFFFFFFF).astype(np.uint32)),
+                           ("bbtab_c2", c2.astype(ctype))]
+                rel2 = self._relative_sources(cell, np.where(two, src2, src))
+                if rel is not None and rel2 is not None:
+                    # bit 31 marks the entries with a second source (the blocked run
+                    # reads no bbtab_src2); the others repeat their first source
+                    rel2 = rel2 | np.where(two, np.uint32(1 << 31), np.uint32(0)).astype(np.uint32)
+                    arrays.append(("bbtab_rel2", rel2))
+                else:
+                    rel = None
             if rel is not None:
                 arrays.append(("bbtab_rel", rel))
             for name, arr in arrays:
@@ -663,7 +748,8 @@ class Simulation:
             return None
         if np.any(np.abs(np.asarray(self.scheme.stencil.get_all_velocities())) > 1):
             return None
-        if self.params.bbtab_bits and not self.params.bbtab_rel:
+        if (self.params.bbtab_bits and not self.params.bbtab_rel) or (self.params.bbtab_src2 and
+                                                                      not self.params.bbtab_rel2):
             return None
         nin = [int(x) for x in self.domain.shape_in]
         cells = nin[0] * nin[1]

@@ -240,17 +240,30 @@ def test_merged_link_launches_bit_identical(hip, dtype, make):
     assert np.array_equal(a.container.F.fields_raw(), b.container.F.fields_raw())
 
 
+def _demo(name, dx):
+    mod = __import__("pylbm_amd.demos", fromlist=["x"])
+    return getattr(mod, name)(dx)[0]
+
+
 @pytest.mark.parametrize("dtype", ["float64", "float32", "float16"])
-@pytest.mark.parametrize("name", ["spheres", "karman", "d1q3_walls", "lid"])
+@pytest.mark.parametrize("name", ["spheres", "karman", "d1q3_walls", "lid", "karman_bouzidi", "poiseuille_demo",
+                                  "lid_demo", "karman_demo", "bend_demo", "poiseuille_3d_demo"])
 def test_folded_links_bit_identical(hip, dtype, name):
     """Bounce-back / anti-bounce-back lists folded into the step's pulls (the default:
     the spheres' zero-value list, the lid cavity's list with the lid's values) and
-    run as link-list launches give the same interior populations, bit for bit,
+    interpolated (Bouzidi) lists folded with their second source, against the same
+    lists run as link-list launches: the same interior populations, bit for bit,
     through one_time_step and through run()."""
     d = {"spheres": lambda: cases.sphere_packing_d3q19(24, 16, 12, seed=5, solid_fraction=0.2),
          "karman": lambda: cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0)),
          "d1q3_walls": lambda: cases.d1q3_advection(16, walls=True),
-         "lid": lambda: cases.lid_cavity_d2q9(24)}[name]()
+         "lid": lambda: cases.lid_cavity_d2q9(24),
+         "karman_bouzidi": lambda: cases.with_bouzidi(cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0))),
+         "poiseuille_demo": lambda: _demo("poiseuille", 1.0 / 32),
+         "lid_demo": lambda: _demo("lid_driven_cavity", 1.0 / 32),
+         "karman_demo": lambda: _demo("karman_vortex_street", 1.0 / 32),
+         "bend_demo": lambda: _demo("bend", 1.0 / 32),
+         "poiseuille_3d_demo": lambda: _demo("poiseuille_3d", 1.0 / 16)}[name]()
     a = Simulation(d, dtype=dtype)
     b = Simulation(d, dtype=dtype, initialize=False)
     b.fold_links = False
@@ -260,12 +273,15 @@ def test_folded_links_bit_identical(hip, dtype, name):
         assert any(x.fused for x in a._dev_links)
     if name == "lid":
         assert a.params.bbtab_rhs   # the lid's values go through the table
+    if name in ("karman_bouzidi", "lid_demo", "karman_demo"):
+        assert a.params.bbtab_src2 and a._wrap   # every interpolated list folded
     for _ in range(3):
         a.one_time_step()
         b.one_time_step()
     a.run(30)
     b.run(30)
     for k in range(a.ns):
+        assert np.isfinite(b.F[k]).all(), k
         assert np.array_equal(a.F[k], b.F[k]), k
 
 
@@ -528,10 +544,14 @@ def test_bouzidi_run_equals_steps(hip, name):
 @pytest.mark.parametrize("dtype", ["float64", "float32-shifted"])
 def test_bouzidi_merged_launches_bit_identical(hip, dtype):
     """One-phase Bouzidi lists grouped with independent lists in one launch
-    (lbm_links_apply_sets) leave the same populations as one launch per list."""
+    (lbm_links_apply_sets) leave the same populations as one launch per list.  The
+    lists stay on the halo path (fold_links off): folded, none of them launches."""
     d = _BOUZIDI["karman"]()
-    a = Simulation(d, dtype=dtype)
+    a = Simulation(d, dtype=dtype, initialize=False)
+    a.fold_links = False
+    a._initialize()
     b = Simulation(d, dtype=dtype, initialize=False)
+    b.fold_links = False
     b.merge_links = False
     b._initialize()
     assert any(ls.merge_prev for ls in a._link_sets[:len(a._active_links)])
@@ -672,6 +692,8 @@ _TB_CASES = {
     "relvel_lid": lambda: _plain_bounce_back(__import__("pylbm_amd.demos", fromlist=["x"]).lid_driven_cavity(1.0 / 32)[0]),
     "vectorial": lambda: cases.d2q5q9_vectorial(24),
     "moving_walls": lambda: _all_walls_moving(12),
+    "karman_bouzidi": lambda: cases.with_bouzidi(cases.karman_d2q9(96, 48, radius=6.0, center=(24.0, 24.0))),
+    "lid_demo": lambda: _demo("lid_driven_cavity", 1.0 / 32),
     "moving_walls_large": lambda: _all_walls_moving(64),
 }
 

@@ -328,7 +328,7 @@ def test_resolve_step_equals_sequential_lists(name):
     seq = F0.copy()
     for lst in lists:
         _apply(seq, lst)
-    slot, org, sg, rh, has, bad = resolve_step([(x[0], x[1], x[2], x[3] if np.any(x[3] != 0) else None)
+    slot, org, sg, rh, has, bad, _, _ = resolve_step([(x[0], x[1], x[2], x[3] if np.any(x[3] != 0) else None)
                                                 for x in lists])
     flat0, flat1 = F0.reshape(-1), seq.reshape(-1)
     val = sg * flat0[org] + np.where(has, rh, 0.0)
@@ -439,7 +439,7 @@ def test_wrap_mode_pulls_equal_halo_pulls(name):
     for lst in lists:
         _apply(seq, lst)
     # wrap mode: folded values where a list stores the pulled slot, else the wrapped read
-    slot, org, sg, rh, has, bad = resolve_step([(x[0], x[1], x[2], x[3] if np.any(x[3] != 0) else None)
+    slot, org, sg, rh, has, bad, _, _ = resolve_step([(x[0], x[1], x[2], x[3] if np.any(x[3] != 0) else None)
                                                 for x in lists])
     n = np.array(lay.n)
     h = np.array([0] * (3 - o.dim) + [int(v) for v in o.h])
@@ -470,3 +470,141 @@ def test_wrap_mode_pulls_equal_halo_pulls(name):
             if s in folded:
                 got[j] = folded[s]
         assert np.array_equal(got, want), (name, q, int(np.sum(got != want)))
+
+
+def test_resolve_step_interpolated_lists():
+    """resolve_step with an interpolated (Bouzidi) list between one-value lists: every
+    slot not flagged equals the sequential application of the lists in order
+    (copies and sign flips resolved into the sources, ca f0 + cb f1 + rhs with
+    contraction off, as the link kernels compute it), bit for bit; a later list that
+    reads an interpolated value, and an interpolation of a value with an rhs, are
+    flagged."""
+    from pylbm_amd.boundary import resolve_step
+    rng = np.random.default_rng(21)
+    f = rng.standard_normal(400)
+    slots = rng.permutation(400)
+    sa, la = slots[:40], slots[200:240]               # copies, sign -1 on half of them
+    sign_a = np.where(np.arange(40) % 2, -1.0, 1.0)
+    sb = slots[40:80]                                   # interpolated stores
+    l0 = np.concatenate([sa[:10], slots[240:270]])      # ten loads of copied values
+    l1 = np.concatenate([slots[270:300], sa[10:20]])
+    ca, cb = rng.uniform(-1, 1, 40), rng.uniform(0, 1, 40)
+    rb = rng.standard_normal(40)
+    sc, lc = slots[80:100], np.concatenate([sb[:5], slots[300:315]])   # five valued reads of interpolations
+    rc = rng.standard_normal(20)
+    sf, lf = slots[120:130], sb[5:15]                                   # signed copies of interpolations
+    sign_f = np.where(np.arange(10) % 2, -1.0, 1.0)
+    sd, ld = slots[100:110], slots[315:325]              # a valued list read by an interpolation
+    rd = rng.standard_normal(10)
+    se = slots[110:120]
+    l0e, l1e = np.concatenate([sd[:3], slots[325:332]]), slots[332:342]
+    cae, cbe = rng.uniform(-1, 1, 10), rng.uniform(0, 1, 10)
+    lists = [(sa, la, sign_a, None), (sb, l0, ca, rb, l1, cb), (sc, lc, np.ones(20), rc),
+             (sd, ld, np.ones(10), rd), (se, l0e, cae, None, l1e, cbe), (sf, lf, sign_f, None)]
+    g = f.copy()
+    g[sa] = sign_a * g[la]
+    v = ca * g[l0] + cb * g[l1]
+    g[sb] = v + rb
+    g[sc] = g[lc] + rc
+    g[sd] = g[ld] + rd
+    g[se] = cae * g[l0e] + cbe * g[l1e]
+    g[sf] = sign_f * g[lf]
+    slot, org, sg, rh, has, bad, org2, c2 = resolve_step(lists)
+    for i, s_ in enumerate(slot):
+        if bad[i]:
+            continue
+        val = sg[i] * f[org[i]]
+        if org2[i] >= 0:
+            val = val + c2[i] * f[org2[i]]
+        if has[i]:
+            val = val + rh[i]
+        assert val == g[s_], (s_, val, g[s_])
+    flagged = set(slot[bad].tolist())
+    assert set(sc[:5].tolist()) <= flagged and set(se[:3].tolist()) <= flagged
+    assert not (set(sb.tolist()) & flagged) and not (set(sf.tolist()) & flagged)
+    assert np.all(org2[np.isin(slot, sf)] >= 0)
+    assert np.all(org2[np.isin(slot, sb)] >= 0)
+
+
+@pytest.mark.parametrize("name", ["karman_bouzidi", "poiseuille", "lid_demo"])
+def test_wrap_mode_pulls_equal_halo_pulls_interpolated(name):
+    """As test_wrap_mode_pulls_equal_halo_pulls, with interpolated (Bouzidi) lists in
+    the step: their device form (boundary.bouzidi_lists) applied in order between the
+    other lists after the wrap of the whole halo, against the step's fold
+    (resolve_step with two-source entries, ghost sources moved to the wrapped
+    interior slot), bit for bit."""
+    from oracle.lbm import Simulation as O
+    from pylbm_amd import demos
+    from pylbm_amd.boundary import _unflat, bouzidi_lists, host_lists, resolve_step
+    d = {"karman_bouzidi": lambda: cases.with_bouzidi(cases.karman_d2q9(40, 20, radius=4.0, center=(10.0, 10.0))),
+         "poiseuille": lambda: demos.poiseuille(1.0 / 16)[0],
+         "lid_demo": lambda: demos.lid_driven_cavity(1.0 / 16)[0]}[name]()
+    o = O(d)
+    dom, pm = _product(d)
+    for m in pm:
+        m.fix_iload()
+    lay = _HostLayout(o.ns, o.F.shape[1:])
+    rng = np.random.default_rng(5)
+    lists, seq_ops = [], []
+    for m in pm:
+        m.rhs = rng.normal(size=m.rhs.shape)
+        if m.kind.bouzidi:
+            st, l0, l1, ca, cb, rhs, two, index, shift = bouzidi_lists(m, lay, np.zeros(o.ns))
+            assert not two
+            lists.append((st, l0, ca, rhs, l1, cb))
+            seq_ops.append(("i", st, l0, l1, ca, cb, rhs))
+        else:
+            x = host_lists(m, lay, None)
+            lists.append((x[0], x[1], x[2], x[3] if np.any(x[3] != 0) else None))
+            seq_ops.append(("c", x[0], x[1], x[2], x[3]))
+    F0 = rng.normal(size=o.F.shape)
+    seq = F0.copy()
+    o.ghost_fill(seq)
+    flat = seq.reshape(-1)
+    for op in seq_ops:
+        if op[0] == "i":
+            _, st, l0, l1, ca, cb, rhs = op
+            flat[st] = ca * flat[l0] + cb * flat[l1] + rhs
+        else:
+            _, st, ld, sign, rhs = op
+            flat[st] = sign * flat[ld] + rhs
+    slot, org, sg, rh, has, bad, org2, c2 = resolve_step(lists)
+    n = np.array(lay.n)
+    h = np.array([0] * (3 - o.dim) + [int(v) for v in o.h])
+    nin = n - 2 * h
+
+    def wrap(c):
+        return [np.where(c[a] < h[a], c[a] + nin[a], np.where(c[a] >= n[a] - h[a], c[a] - nin[a], c[a]))
+                for a in range(3)]
+
+    def wrapped(idx):
+        k, oo, mm, ii = _unflat(idx, lay)
+        w = wrap([np.asarray(oo), np.asarray(mm), np.asarray(ii)])
+        return k * lay.pop_stride + (w[0] * n[1] + w[1]) * lay.row_stride + w[2]
+
+    f0 = F0.reshape(-1)
+    val = sg * f0[wrapped(org)]
+    two_src = org2 >= 0
+    val[two_src] = val[two_src] + c2[two_src] * f0[wrapped(org2[two_src])]
+    val = np.where(has, val + rh, val)
+    folded = dict(zip(slot.tolist(), val.tolist()))
+    vel = np.asarray(o.v)
+    v3 = np.zeros((o.ns, 3), dtype=np.int64)
+    v3[:, 3 - o.dim:] = vel
+    grid = np.stack(np.meshgrid(*[np.arange(h[a], n[a] - h[a]) for a in range(3)], indexing="ij"), axis=0)
+    grid = grid.reshape(3, -1)
+    pulled_bad = 0
+    for q in range(o.ns):
+        src = grid - v3[q][:, None]
+        idx = q * lay.pop_stride + (src[0] * n[1] + src[1]) * lay.row_stride + src[2]
+        want = flat[idx]
+        ws = wrap(src)
+        idx_w = q * lay.pop_stride + (ws[0] * n[1] + ws[1]) * lay.row_stride + ws[2]
+        got = f0[idx_w].copy()
+        bad_slots = set(slot[bad].tolist())
+        for j, s in enumerate(idx.tolist()):
+            if s in folded:
+                got[j] = folded[s]
+                pulled_bad += s in bad_slots
+        assert np.array_equal(got, want), (name, q, int(np.sum(got != want)))
+    assert pulled_bad == 0
