@@ -236,8 +236,18 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(const E* __restrict__ in, 
 // two weight buffers the next layer's weights arrive while the current one computes.  The layer-0 input is read from `xin` and the next step's input is
 // written to `xout` (a different buffer: neighbouring workgroups still read `xin`).
 constexpr int FT_Y = 4, FT_X = 16;
-#ifndef FUSED_PAIR_MASK
-#define FUSED_PAIR_MASK 7  // bit l: layer l of the fused step computes tiles in pairs
+// pixel tiles a wave takes per pass in layer l of the fused step (1, 2 or 4)
+#ifndef FUSED_TP0
+#define FUSED_TP0 2
+#endif
+#ifndef FUSED_TP1
+#define FUSED_TP1 2
+#endif
+#ifndef FUSED_TP2
+#define FUSED_TP2 2
+#endif
+#ifndef FUSED_TP3
+#define FUSED_TP3 1
 #endif
 #ifndef FUSED_PREFETCH
 #define FUSED_PREFETCH 2   // K steps of LDS operands read ahead of the MFMAs
@@ -314,41 +324,64 @@ __device__ __forceinline__ void fused_layer(const E* __restrict__ src, E* __rest
   constexpr int MT = COUT / 16;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
-  // TP = 2: a wave takes a pair of tiles (2 j, 2 j + 1) per pass, pairs j = wv,
-  // wv + FUSED_WAVES, ...: each weight read feeds two MFMAs and the MFMA pipe has 2 MT
-  // independent chains; TP = 1: one tile per pass (more waves busy on small layers)
-  constexpr int TP = ((FUSED_PAIR_MASK >> LAYER) & 1) ? 2 : 1;
+  // A wave takes TP tiles (TP j .. TP j + TP - 1) per pass, groups j = wv,
+  // wv + FUSED_WAVES, ...: each weight read feeds TP MFMAs and the MFMA pipe has
+  // TP MT independent chains; fewer tiles per pass keep more waves busy on small layers
+  constexpr int TP = LAYER == 0 ? FUSED_TP0 : LAYER == 1 ? FUSED_TP1 : LAYER == 2 ? FUSED_TP2 : FUSED_TP3;
+  static_assert(TP == 1 || TP == 2 || TP == 4, "tiles per pass");
   constexpr int GROUPS = (TILES + TP - 1) / TP;
   for (int j = wv; j < GROUPS; j += FUSED_WAVES) {
-    int pcs[2], prs[2], pqs[2];
-    bool st[2];
+    int pcs[TP], prs[TP], pqs[TP];
+    bool st[TP];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int p = (TP * j + (TP == 2 ? u : 0)) * 16 + r;
+    for (int u = 0; u < TP; ++u) {
+      const int p = (TP * j + u) * 16 + r;
       st[u] = p < NPIX;
       pcs[u] = st[u] ? p : NPIX - 1;  // padding lanes compute a valid pixel, store nothing
       prs[u] = pcs[u] / WO;
       pqs[u] = pcs[u] % WO;
     }
-    float4v acc[2][MT];
+    // epilogue operands that do not depend on the MFMAs, loaded before the K loop so
+    // that their memory latency hides behind it: the bias and, in the last layer,
+    // the FP32 state the increment is added to (every pixel index is valid)
+    float4v bq[MT];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int mt = 0; mt < MT; ++mt) __builtin_memcpy(&bq[mt], bias + mt * 16 + g * 4, 16);
+    float4v svp[TP][LAYER == 3 ? MT : 1];
+    int64_t fps[TP][LAYER == 3 ? MT : 1];
+    if constexpr (LAYER == 3) {
+#pragma unroll
+      for (int u = 0; u < TP; ++u)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          // core pixel (y0 + pr, x0 + pq); channels c0..c0+3 = fields of fine pixel (py, px)
+          const int c0 = mt * 16 + g * 4;
+          const int py = c0 >> 4, pxo = (c0 >> 2) & 3;
+          fps[u][mt] = ((int64_t)n * 4 * H + 4 * (y0 + prs[u]) + py) * (4 * W) + 4 * (x0 + pqs[u]) + pxo;
+          __builtin_memcpy(&svp[u][mt], ap.state_in + fps[u][mt] * 4, 16);
+        }
+    }
+    (void)svp; (void)fps;
+    float4v acc[TP][MT];
+#pragma unroll
+    for (int u = 0; u < TP; ++u)
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) acc[u][mt] = float4v{0.f, 0.f, 0.f, 0.f};
     // K steps ks = t * KC + kc, software-pipelined: the operands of step ks + 1 are
     // read from LDS while the MFMAs of step ks issue
     constexpr int KC = CIN / 32, NK = 9 * KC;
-    const E* bbase0 = src + (prs[0] * WI + pqs[0]) * CIPI + g * 8;
-    const E* bbase1 = src + (prs[1] * WI + pqs[1]) * CIPI + g * 8;
+    const E* bbase[TP];
+#pragma unroll
+    for (int u = 0; u < TP; ++u) bbase[u] = src + (prs[u] * WI + pqs[u]) * CIPI + g * 8;
     const E* abase = sw + r * KPP + g * 8;
     // ring of FUSED_PREFETCH + 1 operand sets
     constexpr int D = FUSED_PREFETCH;
-    half8 bop[D + 1][2], aop[D + 1][MT];
-    auto load_ops = [&](int ks, half8 (&b)[2], half8 (&a)[MT]) {
+    half8 bop[D + 1][TP], aop[D + 1][MT];
+    auto load_ops = [&](int ks, half8 (&b)[TP], half8 (&a)[MT]) {
       const int t = ks / KC, kc = ks % KC;
       const int boff = ((t / 3) * WI + t % 3) * CIPI + kc * 32;
-      __builtin_memcpy(&b[0], bbase0 + boff, 16);
-      if constexpr (TP == 2) __builtin_memcpy(&b[1], bbase1 + boff, 16);
+#pragma unroll
+      for (int u = 0; u < TP; ++u) __builtin_memcpy(&b[u], bbase[u] + boff, 16);
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) __builtin_memcpy(&a[mt], abase + mt * 16 * KPP + t * CIN + kc * 32, 16);
     };
@@ -360,8 +393,8 @@ __device__ __forceinline__ void fused_layer(const E* __restrict__ src, E* __rest
       const int c = ks % (D + 1);
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        acc[0][mt] = Elem<E>::mfma(aop[c][mt], bop[c][0], acc[0][mt]);
-        if constexpr (TP == 2) acc[1][mt] = Elem<E>::mfma(aop[c][mt], bop[c][1], acc[1][mt]);
+#pragma unroll
+        for (int u = 0; u < TP; ++u) acc[u][mt] = Elem<E>::mfma(aop[c][mt], bop[c][u], acc[u][mt]);
       }
     }
 #pragma unroll
@@ -375,21 +408,18 @@ __device__ __forceinline__ void fused_layer(const E* __restrict__ src, E* __rest
           half4 ab;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const float z = acc[u][mt][q] + bias[c0 + q];
+            const float z = acc[u][mt][q] + bq[mt][q];
             ab[q] = silu_e<E>(z);
           }
           __builtin_memcpy(dst + pc * CIPO + c0, &ab, 8);
         } else {
-          // core pixel (y0 + pr, x0 + pq); channels c0..c0+3 = fields of fine pixel (py, px)
           const int y = y0 + pr, x = x0 + pq;
-          const int py = c0 >> 4, pxo = (c0 >> 2) & 3;
-          const int64_t fp = ((int64_t)n * 4 * H + 4 * y + py) * (4 * W) + 4 * x + pxo;
-          float4v sv;
-          __builtin_memcpy(&sv, ap.state_in + fp * 4, 16);
+          const int64_t fp = fps[u][mt];
+          float4v sv = svp[u][mt];
           half4 xb;
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
-            sv[q] += (acc[u][mt][q] + bias[c0 + q]) * ap.dscale[q];
+            sv[q] += (acc[u][mt][q] + bq[mt][q]) * ap.dscale[q];
             xb[q] = input_e<E>(sv[q], ap.shift[q], ap.inv_scale[q]);
           }
           xb[3] = (E)0.f;
@@ -433,10 +463,12 @@ __global__ void __launch_bounds__(64 * FUSED_WAVES) fused_step_kernel(const E* _
     });
   }
   dma_weights<64, C>(w0, sw0);
-  if constexpr (G::DB) dma_weights<C, C>(w1, sw1);
   dma_wait();
+  // layer 1's weights land while layer 0 computes
+  if constexpr (G::DB) dma_weights<C, C>(w1, sw1);
   __syncthreads();
   fused_layer<E, C, 0, 64, C>(bufA, bufB, sw0, b0, n, y0, x0, H, W, ap);
+  if constexpr (G::DB) dma_wait();
   __syncthreads();
   if constexpr (G::DB) {
     dma_weights<C, C>(w2, sw0);

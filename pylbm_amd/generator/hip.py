@@ -1012,6 +1012,18 @@ __device__ __forceinline__ int wrap_near(int x, int n) {
   return x;
 }
 
+// Population planes of the region lie TB_PLANE<S> cells apart: a compile-time
+// stride, so that the plane offsets of the pulls and stores become immediates of
+// the LDS instructions.  It is the most cells the two copies of the populations
+// plus the mask / table words fit in 160 KiB with, and keeps the last plane within
+// the 16-bit immediate offset.
+template <typename S>
+__host__ __device__ constexpr int tb_plane() {
+  constexpr int a = (160 * 1024) / (2 * NS * (int)sizeof(S) + 12);
+  constexpr int b = NS > 1 ? 65528 / ((NS - 1) * (int)sizeof(S)) : a;
+  return a < b ? a : b;
+}
+
 template <typename S>
 __global__ void __launch_bounds__(TB_THREADS)
 tb_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Codec<S>::C> a,
@@ -1019,15 +1031,16 @@ tb_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Cod
   using C = typename Codec<S>::C;
   constexpr bool DEV = Codec<S>::kind != LBM_F64;
   extern __shared__ uint4 tb_smem[];
-  const int RW = t.tm + 2 * t.nt, RI = t.ti + 2 * t.nt, RC = RW * RI;
+  const int RW = t.tm + 2 * t.nt, RI = t.ti + 2 * t.nt, RN = RW * RI;
+  constexpr int RC = tb_plane<S>();   // plane stride (cells), RN <= RC
   S* cur = reinterpret_cast<S*>(tb_smem);
   S* nxt = cur + NS * RC;
   // constant over the launch: the region's mask words (folded links) and, with
   // table links, every region cell's table bits and first entry (loaded with the
   // populations, so a step reads its entries after one round trip instead of two)
   uint32_t* msk = reinterpret_cast<uint32_t*>(cur + 2 * NS * RC);
-  uint32_t* tbits = msk + RC;
-  uint32_t* tbase_l = tbits + RC;
+  uint32_t* tbits = msk + RN;
+  uint32_t* tbase_l = tbits + RN;
   const bool table = TB_MASK && a.bbmask && a.bbtab_bits;
   (void)msk; (void)tbits; (void)tbase_l; (void)table;
   const int m0 = (int)(blockIdx.x / t.tiles_i) * t.tm - t.nt;
@@ -1038,7 +1051,7 @@ tb_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Cod
   const C* omega = a.omega;
   const C* cref = a.cref;
   (void)cref;
-  for (int c = threadIdx.x; c < RC; c += blockDim.x) {
+  for (int c = threadIdx.x; c < RN; c += blockDim.x) {
     const int lr = c / RI;
     const int gm = wrap_near(m0 + lr, a.nin[1]), gi = wrap_near(i0 + c - lr * RI, a.nin[2]);
     const int64_t src = (int64_t)(a.halo[0] * a.n1 + gm + a.halo[1]) * a.rs + a.lead + gi + a.halo[2];
@@ -1059,18 +1072,23 @@ tb_step(const S* __restrict__ f, S* __restrict__ fn, const StepArgs<typename Cod
     (void)tt;
     const int WC = RI - 2 * s, HC = RW - 2 * s;
     // the cells of the shrunk region in row-major order, blockDim.x apart: the
-    // (row, column) of the next one follow by adding (dq, dr) with a carry
+    // (row, column) of the next one follow by adding (dq, dr) with a carry, and its
+    // region index by adding dl, plus dc on a carry (no multiply per cell)
     const int dq = (int)blockDim.x / WC, dr = (int)blockDim.x - dq * WC;
     int rr = (int)threadIdx.x / WC, qq = (int)threadIdx.x - rr * WC;
+    const int dl = dq * RI + dr, dc = RI - WC;
+    int lnext = (s + rr) * RI + s + qq;
     for (int c = threadIdx.x; c < WC * HC; c += blockDim.x) {
       const int lr = s + rr, lc = s + qq;
+      const int lcell = lnext;
       qq += dr;
       rr += dq;
+      lnext += dl;
       if (qq >= WC) {
         qq -= WC;
         ++rr;
+        lnext += dc;
       }
-      const int lcell = lr * RI + lc;
       const int gm = wrap_near(m0 + lr, a.nin[1]), gi = wrap_near(i0 + lc, a.nin[2]);
       const int o = a.halo[0], m = gm + a.halo[1], ii = gi + a.halo[2];
       (void)o; (void)m; (void)ii;
@@ -1133,7 +1151,9 @@ static int tb_launch(void* f, void* fn, const lbm_layout* L, lbm_params* p, int 
   const int tiles_m = (a.nin[1] + tm - 1) / tm;
   const size_t region = (size_t)(tm + 2 * nt) * (ti + 2 * nt);
   const bool mask = TB_MASK && p->bbmask, table = mask && p->bbtab_bits;
-  const size_t lds = region * (2 * NS * sizeof(S) + (mask ? 4 : 0) + (table ? 8 : 0));
+  LBM_REQUIRE(region <= (size_t)tb_plane<S>(), "tile region of %%zu cells exceeds the %%d cells of a plane",
+              region, tb_plane<S>());
+  const size_t lds = 2 * NS * sizeof(S) * (size_t)tb_plane<S>() + region * ((mask ? 4 : 0) + (table ? 8 : 0));
   LBM_REQUIRE(lds <= 160 * 1024, "tile needs %%zu bytes of LDS", lds);
   LBM_CHECK_HIP(hipFuncSetAttribute((const void*)tb_step<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));

@@ -737,9 +737,9 @@ class Simulation:
         None: 2D schemes on the wrap path (periodic, or every list folded with its
         table sources within one cell) whose grid is small enough that a step does not
         fill the GPU (at most 2^21 cells).  The tile and its halo live in LDS: two
-        copies of ns populations per cell, plus the mask word with folded links and
-        the table bits and first entry with table links, within the 160 KiB of a
-        CU.  The preferred shapes were measured on MI355X
+        copies of ns populations per cell in planes of a fixed size (the most cells
+        that fit the 160 KiB of a CU with the mask and table words), and the region
+        has to fit one plane.  The preferred shapes were measured on MI355X
         (D2Q9, 128^2 to 1024 x 256): 6 steps of 32 x 32 tiles for the 4- and 2-byte
         formats, 4 steps of 16 x 32 tiles for FP64, 6 steps of 8 x 16 tiles for FP64
         grids of at most 2^15 cells (more tiles than CUs matter more there than the
@@ -756,13 +756,15 @@ class Simulation:
         if cells > (1 << 21):
             return None
         es = self.container.F.layout.elem_size
-        table = bool(self.params.bbmask and self.params.bbtab_bits)
-        per_cell = 2 * self.ns * es + (4 if self.params.bbmask else 0) + (8 if table else 0)
+        # the kernel's population planes hold this many cells (tb_plane in the
+        # generated module): every region has to fit one plane
+        plane = min((160 * 1024) // (2 * self.ns * es + 12),
+                    65528 // ((self.ns - 1) * es) if self.ns > 1 else 1 << 30)
         if es == 8:
             nt, tm, ti = (6, 8, 16) if cells <= (1 << 15) else (4, 16, 32)
         else:
             nt, tm, ti = 6, 32, 32
-        while (tm + 2 * nt) * (ti + 2 * nt) * per_cell > 160 * 1024:
+        while (tm + 2 * nt) * (ti + 2 * nt) > plane:
             if ti > 16:
                 ti //= 2
             elif tm > 8:
