@@ -734,12 +734,18 @@ static int step_launch(const void* f, void* fn, const lbm_layout* L, const lbm_p
   if (rc) return rc;
   constexpr int V = vec_cells<S>();
   // V cells per thread only where that still leaves >= 2^20 threads: small grids
-  // (a 512^2 plane) need the waves of one cell per thread to hide memory latency
+  // (a 512^2 plane) need the waves of one cell per thread to hide memory latency.
+  // Only the wrap path without folded links takes V cells: with a mask word per
+  // cell the per-cell mask branches of a V-cell thread cost more than the wider
+  // loads save, and the halo path's pulls straddle more cache lines (MI355X, 512 x
+  // 256 x 256 D3Q19 sphere packing FP32: 0.90 ms per step with one cell per thread,
+  // 0.93 with 2, 0.98 with 4; 256^3 halo path FP32 with its ghost fill: 0.514 ms
+  // with one, 0.534 with 4).
   const int64_t cells = (int64_t)a.nin[0] * a.nin[1] * a.nin[2];
   const int want = p ? p->vec_cells : 0;
   LBM_REQUIRE(want == 0 || want == 1 || (want == V && a.nin[2] %% V == 0),
               "vec_cells %%d not available (%%d cells per thread, contiguous extent %%d)", want, V, a.nin[2]);
-  const bool auto_vec = sizeof(S) != 2 || LBM_VEC16_AUTO;
+  const bool auto_vec = (sizeof(S) != 2 || LBM_VEC16_AUTO) && fold && a.bbmask == nullptr;
   const int vec = want ? want : ((auto_vec && a.nin[2] %% V == 0 && cells / V >= ((int64_t)1 << 20)) ? V : 1);
   a.remap = (p && p->xcd_remap >= 0) ? (p->xcd_remap != 0) : !fold;
   const int threads = a.nin[2] / vec;
