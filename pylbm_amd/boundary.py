@@ -343,21 +343,9 @@ def host_lists(method, layout, wshift, flat=None):
     return st, ld, sign, rhs, conflict, order
 
 
-def bouzidi_lists(method, layout, wshift):
-    """The lists an interpolated (Bouzidi) method runs with:
-    (store, load0, load1, ca, cb, rhs, two_phase, index, shift) with flat element
-    indices of layout, f[store] = ca f[load0] + cb f[load1] + rhs.
-
-    Bouzidi bounce-back reads a copy of f taken before its loop (pylbm/boundary.py:
-    545-549), so every load sees the old value and the last store of a slot wins:
-    earlier duplicate stores are dropped and, when a load hits a stored slot, the
-    list runs two-phase (all values gathered before any store).  Bouzidi
-    anti-bounce-back reads f itself in loop order: a load of a slot that an
-    earlier link of the loop stored would need that link's new value, which this
-    form does not carry (NotImplementedError; it needs a fluid gap of one cell);
-    loads of slots stored later see the old value, as two-phase gives them.
-    index: link positions of the method kept, in run order (rhs[index] + shift is
-    the device rhs); shift: the rest-population term of deviation formats."""
+def _bouzidi_flat(method, layout):
+    """Store, first and second load (flat element indices of layout) and the
+    per-link s of an interpolated method, in its link order."""
     rows_s = method._store_rows()
     l0r, l1r = [(a.T if method._fixed else a) for a in method.iload[:2]]
     dim = rows_s.shape[0] - 1
@@ -366,26 +354,25 @@ def bouzidi_lists(method, layout, wshift):
             c = np.asarray(rows[1 + d])
             if c.size and (c.min() < 0 or c.max() >= layout.n[3 - dim + d]):
                 raise ValueError("%s: a link slot lies outside the halo-inclusive grid" % method.name)
-    st, l0, l1 = _flat(rows_s, layout), _flat(l0r, layout), _flat(l1r, layout)
-    s = np.asarray(method.s, dtype=np.float64)
+    return (_flat(rows_s, layout), _flat(l0r, layout), _flat(l1r, layout),
+            np.asarray(method.s, dtype=np.float64))
+
+
+def _earlier_store_loads(st, l0, l1):
+    """Whether a link loads a slot that an earlier link of the list stores."""
     n = st.size
-    if not method.kind.reads_copy and n:
-        us, first = np.unique(st, return_index=True)
-        links = np.concatenate([np.arange(n), np.arange(n)])
-        loads = np.concatenate([l0, l1])
-        pos = np.minimum(np.searchsorted(us, loads), us.size - 1)
-        hit = us[pos] == loads
-        if np.any(first[pos[hit]] < links[hit]):
-            raise NotImplementedError("%s: a link loads a slot an earlier link of the list stores"
-                                      % method.name)
-    # the last store of each slot
-    _, last_rev = np.unique(st[::-1], return_index=True)
-    index = np.sort(n - 1 - last_rev)
-    st, l0, l1, s = st[index], l0[index], l1[index], s[index]
-    two = bool(np.isin(np.concatenate([l0, l1]), st).any())
-    if not two and st.size > 1:
-        order = np.argsort(st, kind="stable")
-        index, st, l0, l1, s = index[order], st[order], l0[order], l1[order], s[order]
+    if not n:
+        return False
+    us, first = np.unique(st, return_index=True)
+    links = np.concatenate([np.arange(n), np.arange(n)])
+    loads = np.concatenate([l0, l1])
+    pos = np.minimum(np.searchsorted(us, loads), us.size - 1)
+    hit = us[pos] == loads
+    return bool(np.any(first[pos[hit]] < links[hit]))
+
+
+def _bouzidi_form(method, layout, wshift, index, st, l0, l1, s, two):
+    """(store, load0, load1, ca, cb, rhs, two_phase, index, shift) of the links index."""
     ca = method.kind.sign * s
     cb = 1.0 - s
     shift = np.zeros(st.size)
@@ -397,15 +384,84 @@ def bouzidi_lists(method, layout, wshift):
     return st, l0, l1, ca, cb, rhs, two, index, shift
 
 
+def bouzidi_lists(method, layout, wshift):
+    """The lists an interpolated (Bouzidi) method runs with:
+    (store, load0, load1, ca, cb, rhs, two_phase, index, shift) with flat element
+    indices of layout, f[store] = ca f[load0] + cb f[load1] + rhs.
+
+    Bouzidi bounce-back reads a copy of f taken before its loop (pylbm/boundary.py:
+    545-549), so every load sees the old value and the last store of a slot wins:
+    earlier duplicate stores are dropped and, when a load hits a stored slot, the
+    list runs two-phase (all values gathered before any store).  Bouzidi
+    anti-bounce-back reads f itself in loop order: a load of a slot that an
+    earlier link of the loop stored needs that link's new value, which one list
+    does not carry (NotImplementedError here; bouzidi_levels splits such a list);
+    loads of slots stored later see the old value, as two-phase gives them.
+    index: link positions of the method kept, in run order (rhs[index] + shift is
+    the device rhs); shift: the rest-population term of deviation formats."""
+    st, l0, l1, s = _bouzidi_flat(method, layout)
+    n = st.size
+    if not method.kind.reads_copy and _earlier_store_loads(st, l0, l1):
+        raise NotImplementedError("%s: a link loads a slot an earlier link of the list stores"
+                                  % method.name)
+    # the last store of each slot
+    _, last_rev = np.unique(st[::-1], return_index=True)
+    index = np.sort(n - 1 - last_rev)
+    st, l0, l1, s = st[index], l0[index], l1[index], s[index]
+    two = bool(np.isin(np.concatenate([l0, l1]), st).any())
+    if not two and st.size > 1:
+        order = np.argsort(st, kind="stable")
+        index, st, l0, l1, s = index[order], st[order], l0[order], l1[order], s[order]
+    return _bouzidi_form(method, layout, wshift, index, st, l0, l1, s, two)
+
+
+def bouzidi_levels(method, layout, wshift):
+    """The interpolated method as lists run one after the other, each in the form of
+    bouzidi_lists.  One list, unless Bouzidi anti-bounce-back has a link that loads
+    a slot an earlier link of its loop stored (pylbm's in-place loop, pylbm/
+    boundary.py:743-760: a one-cell gap whose nearer wall is the upper one).  Then
+    the links are split into dependency levels, in link order: a link's level is
+    above that of the latest earlier link storing a slot it loads (it reads that
+    value), above every earlier link loading the slot it stores (they read the old
+    value) and above every earlier link storing that slot (the last store wins).
+    The links of one level touch no slot another one of the level stores, so each
+    level runs one-phase, and the levels in order give the loop's result."""
+    if method.kind.reads_copy:
+        return [bouzidi_lists(method, layout, wshift)]
+    st, l0, l1, s = _bouzidi_flat(method, layout)
+    if not _earlier_store_loads(st, l0, l1):
+        return [bouzidi_lists(method, layout, wshift)]
+    n = st.size
+    level = np.zeros(n, dtype=np.int64)
+    stored, read = {}, {}          # slot -> level of its latest store / highest read
+    for j in range(n):
+        a, b, x = int(l0[j]), int(l1[j]), int(st[j])
+        lv = max(stored.get(a, -1), stored.get(b, -1), read.get(x, -1), stored.get(x, -1)) + 1
+        level[j] = lv
+        stored[x] = lv
+        read[a] = max(read.get(a, -1), lv)
+        read[b] = max(read.get(b, -1), lv)
+    out = []
+    for lv in range(int(level.max()) + 1):
+        index = np.nonzero(level == lv)[0]
+        order = np.argsort(st[index], kind="stable")
+        index = index[order]
+        out.append(_bouzidi_form(method, layout, wshift, index, st[index], l0[index], l1[index], s[index],
+                                 False))
+    return out
+
+
 class DeviceLinks:
     """Device arrays of one method for a given population layout."""
 
-    def __init__(self, rt, method, layout, wshift, flat=None):
+    def __init__(self, rt, method, layout, wshift, flat=None, level=0):
         """flat=(store, load): a plain copy list f[store] = f[load] given as flat
-        element indices (method is then only a name)."""
+        element indices (method is then only a name).  level: which list of
+        bouzidi_levels an interpolated method's object runs."""
         self.rt = rt
         self.method = method
         self.layout = layout
+        self.level = level
         self.bouzidi = flat is None and bool(getattr(method.kind, "bouzidi", False))
         self.d_load1 = self.d_ca = self.d_cb = None
         if self.bouzidi:
@@ -429,8 +485,12 @@ class DeviceLinks:
         if self.d_work:
             self._ptrs.append(self.d_work)
 
+    def bouzidi_form(self, layout, wshift):
+        """This object's interpolated list in the form of bouzidi_lists, for layout."""
+        return bouzidi_levels(self.method, layout, wshift)[self.level]
+
     def _init_bouzidi(self, method, layout, wshift):
-        st, l0, l1, ca, cb, rhs, self.conflict, self.order, self._shift = bouzidi_lists(method, layout, wshift)
+        st, l0, l1, ca, cb, rhs, self.conflict, self.order, self._shift = self.bouzidi_form(layout, wshift)
         self.n = st.size
         self.h_store, self.h_load = st, np.concatenate([l0, l1])
         self.h_sign, self.h_rhs = None, None

@@ -41,7 +41,10 @@ kernels combine stored deviations ``f - w``; that term is applied even when
 are exactly its own).  For the Bouzidi routines ``dist`` is the FP64 device
 array of pylbm's per-link ``s``, ``iload1`` the second loads, and a list that
 loads slots it stores is given with ``work`` (pylbm's ``fcopy`` semantics: every
-value is gathered before any store; ``fcopy`` itself is not read).  The
+value is gathered before any store; ``fcopy`` itself is not read).  One call is
+one launch: an anti-bounce-back list one of whose links loads a slot an earlier
+link stores (pylbm's in-place loop reads that new value) is given as the lists
+of ``pylbm_amd.boundary.bouzidi_levels``, one call per level in order.  The
 coefficient and rhs buffers the module makes are reused across calls (grown to
 the largest ncond) and their contents recomputed on every call, so new values
 behind the same addresses are never served stale; they are released by

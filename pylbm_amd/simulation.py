@@ -26,7 +26,7 @@ import numpy as np
 import sympy as sp
 
 from . import runtime as rtm
-from .boundary import (Boundary, DeviceLinks, bouzidi_lists, host_lists, merge_flags, resolve_step,
+from .boundary import (Boundary, DeviceLinks, bouzidi_levels, host_lists, merge_flags, resolve_step,
                        unwritten_ghost_reads)
 from .boundary import _unflat as _unflat_fn
 from .domain import Domain
@@ -277,8 +277,15 @@ class Simulation:
             self._prepare_rhs(method)
             method.fix_iload()
             method.set_rhs()
-        self._dev_links = [DeviceLinks(self.rt, m, self.container.F.layout, self.wshift)
-                           for m in self.bc.methods]
+        # one device list per method; an interpolated method whose loop reads values
+        # stored earlier in the same loop runs as several lists (bouzidi_levels)
+        layout = self.container.F.layout
+        self._method_devs = []
+        for m in self.bc.methods:
+            nlev = len(bouzidi_levels(m, layout, self.wshift)) if getattr(m.kind, "bouzidi", False) else 1
+            self._method_devs.append((m, [DeviceLinks(self.rt, m, layout, self.wshift, level=k)
+                                          for k in range(nlev)]))
+        self._dev_links = [d for _, devs in self._method_devs for d in devs]
         if self._ghost_mode == 1:
             st, ld = unwritten_ghost_reads(self.domain, self.scheme.stencil, self.bc.methods,
                                            self.container.F.layout)
@@ -354,7 +361,7 @@ class Simulation:
                 lists.append((d.h_store, d.h_load, d.h_sign, d.h_rhs))
                 continue
             # an interpolated list: two sources per link, one phase only
-            st, l0, l1, ca, cb, rhs, two, index, shift = bouzidi_lists(d.method, L, self.wshift)
+            st, l0, l1, ca, cb, rhs, two, index, shift = d.bouzidi_form(L, self.wshift)
             if two:
                 return False
             lists.append((st, l0, ca, rhs if d.d_rhs is not None else None, l1, cb))
@@ -459,7 +466,7 @@ class Simulation:
         an earlier list stores (the ghost copy aside: it stores the periodic wrap the
         folded loads read anyway)."""
         layout = self.container.F.layout
-        st, l0, l1, ca, cb, rhs, two, index, shift = bouzidi_lists(d.method, layout, self.wshift)
+        st, l0, l1, ca, cb, rhs, two, index, shift = d.bouzidi_form(layout, self.wshift)
         if two or st.size == 0:
             return None
         L, n, h, nin = self._fold_layout()
@@ -631,8 +638,9 @@ class Simulation:
         self._m2f(m, f)
         return f.fields().reshape(self.ns, -1)
 
-    def _update_feq(self, method, dev):
-        """Time-dependent boundary values (pylbm/boundary.py:307-321)."""
+    def _update_feq(self, method, devs):
+        """Time-dependent boundary values (pylbm/boundary.py:307-321) into the
+        method's device lists."""
         if not method._tbc:
             return
         for key, func, args, cols, shape in method._tbc:
@@ -640,7 +648,8 @@ class Simulation:
             func(None, _MomentView(host, self.scheme.consm), self.t, *args)
             method.feq[:, cols] = self._feq(host, shape)
         method.set_rhs()
-        dev.update_rhs(method.rhs, self.wshift)
+        for dev in devs:
+            dev.update_rhs(method.rhs, self.wshift)
 
     # ---------------------------------------------------------- stepping
     def boundary_condition(self):
@@ -651,14 +660,13 @@ class Simulation:
         self._check(fn(ctypes.c_void_p(f.ptr), ctypes.byref(f.layout), ctypes.byref(self.vel),
                        self.rt.stream), "ghost fill")
         devs = self._dev_links
-        if len(devs) > len(self.bc.methods):
-            if not devs[0].fused:   # the ghost copy list, unless folded into the step
-                devs[0].apply(f.ptr)
-            devs = devs[1:]
-        for method, dev in zip(self.bc.methods, devs):
-            self._update_feq(method, dev)
-            if not dev.fused:
-                dev.apply(f.ptr)
+        if devs and isinstance(devs[0].method, str) and not devs[0].fused:
+            devs[0].apply(f.ptr)   # the ghost copy list, unless folded into the step
+        for method, mdevs in self._method_devs:
+            self._update_feq(method, mdevs)
+            for dev in mdevs:
+                if not dev.fused:
+                    dev.apply(f.ptr)
 
     def _fused(self):
         F, Fn = self.container.F, self.container.Fnew

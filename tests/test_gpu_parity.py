@@ -518,6 +518,20 @@ def test_bouzidi_fp64_parity(hip, name):
     _compare(sim, ref, syms, TOL64)
 
 
+@pytest.mark.parametrize("use_run", [False, True])
+def test_bouzidi_anti_bounce_back_levels_fp64_parity(hip, use_run):
+    """A one-row gap whose upper wall is the nearer one: Bouzidi anti-bounce-back
+    links load slots that earlier links of the same loop stored (pylbm's in-place
+    loop, boundary.py:743-760).  The list runs as its dependency levels, one launch
+    each, and matches the oracle's sequential loop at the FP64 bar."""
+    d = cases.poiseuille_d2q9(ny=5, y_low=0.9, y_high=1.8, method=bc.BouzidiAntiBounceBack)
+    sim, ref = Simulation(d), Oracle(d)
+    bz = [dl for dl in sim._dev_links if dl.bouzidi]
+    assert len(bz) > 1 and sorted(dl.level for dl in bz) == list(range(len(bz)))
+    _march(sim, ref, 30, use_run=use_run)
+    _compare(sim, ref, [cases.RHO, cases.QX, cases.QY], TOL64)
+
+
 @pytest.mark.parametrize("dtype", ["float32-shifted", "float16"])
 def test_bouzidi_deviation_formats(hip, dtype):
     """Bouzidi lists with the rest-population shift in rhs

@@ -389,6 +389,192 @@ def test_bouzidi_lists_equal_sequential_loop(name):
     assert checked
 
 
+@pytest.mark.parametrize("gap", [(0.9, 1.8), (0.7, 1.6), (1.3, 2.4)])
+def test_bouzidi_anti_bounce_back_levels_equal_sequential_loop(gap):
+    """Bouzidi anti-bounce-back reads f in loop order: in a one-row gap whose upper
+    wall is the nearer one, the upper links (populations pointing down, later in the
+    velocity order) load slots of the lower solid row that the lower links stored
+    earlier.  One list cannot carry those values (bouzidi_lists raises); the
+    dependency levels of bouzidi_levels, each applied one-phase in order, equal the
+    oracle's in-place loop bit for bit, and every level is free of internal
+    conflicts (no slot stored twice, no load of a slot the level stores)."""
+    from oracle.boundary import _apply_bouzidi
+    from pylbm_amd.boundary import _flat, bouzidi_levels, bouzidi_lists
+    from pylbm_amd.runtime import LBM_F64
+    from pylbm_amd.storage import layout_elements, make_layout
+    d = cases.poiseuille_d2q9(ny=5, y_low=gap[0], y_high=gap[1], method=bc.BouzidiAntiBounceBack)
+    dom, pm = _product(d)
+    _, om = _oracle(d)
+    ns = int(dom.stencil.nv_ptr[-1])
+    shape = tuple(dom.shape_halo)
+    L = make_layout(ns, shape, list(dom.stencil.vmax), LBM_F64)
+    rng = np.random.default_rng(5)
+    F = rng.standard_normal((ns,) + shape)
+    every = np.array(np.unravel_index(np.arange(F.size), F.shape))
+    pos = _flat(every, L)
+    flat = np.full(layout_elements(L), np.nan)
+    flat[pos] = F.ravel()
+    p, o = pm[0], om[0]
+    rhs = rng.standard_normal(p.ncond)
+    p.rhs, o.rhs = rhs.copy(), rhs.copy()
+    levels = bouzidi_levels(p, L, np.zeros(ns))
+    if gap == (1.3, 2.4):
+        assert len(levels) == 1       # the nearer wall is the lower one: one list
+    else:
+        assert len(levels) > 1
+        with pytest.raises(NotImplementedError):
+            bouzidi_lists(p, L, np.zeros(ns))
+    covered = np.zeros(p.ncond, dtype=int)
+    for st, l0, l1, ca, cb, r, two, index, shift in levels:
+        if len(levels) > 1:
+            assert not two and np.unique(st).size == st.size
+            assert not np.isin(np.concatenate([l0, l1]), st).any()
+        covered[index] += 1
+        vals = ca * flat[l0] + cb * flat[l1] + r
+        flat[st] = vals
+    if len(levels) > 1:
+        assert np.all(covered == 1)   # every link runs exactly once
+    _apply_bouzidi(F, o)
+    assert np.array_equal(flat[pos], F.ravel())
+
+
+@pytest.mark.parametrize("name", ["spheres", "lid", "d1q3_walls", "karman", "vectorial", "neumann_periodic"])
+def test_merged_link_groups_equal_sequential(name):
+    """The step's lists (ghost copy list first) applied one after another equal the
+    groups merge_flags forms, each applied as one gather-then-scatter, bit for bit."""
+    from oracle.lbm import Simulation as O
+    from pylbm_amd.boundary import host_lists, merge_flags, unwritten_ghost_reads
+    d = {"spheres": lambda: cases.sphere_packing_d3q19(20, 14, 12, seed=3, solid_fraction=0.25),
+         "lid": lambda: cases.lid_cavity_d2q9(12),
+         "d1q3_walls": lambda: cases.d1q3_advection(10, walls=True),
+         "karman": lambda: cases.karman_d2q9(40, 20, radius=4.0, center=(10.0, 10.0)),
+         "vectorial": lambda: cases.d2q5q9_vectorial(12, walls=True),
+         "neumann_periodic": _neumann_periodic}[name]()
+    o = O(d)
+    dom, pm = _product(d)
+    for m in pm:
+        m.fix_iload()
+    lay = _HostLayout(o.ns, o.F.shape[1:])
+    lists = []
+    st, ld = unwritten_ghost_reads(dom, dom.stencil, pm, lay)
+    if st.size:
+        lists.append(host_lists("ghost_copy", lay, None, flat=(st, ld)))
+    rng = np.random.default_rng(1)
+    for m in pm:
+        m.rhs = rng.normal(size=m.rhs.shape)
+        lists.append(host_lists(m, lay, None))
+    flags = merge_flags([(x[0], x[1], x[4]) for x in lists])
+    F0 = rng.normal(size=o.F.shape)
+    seq = F0.copy()
+    for lst in lists:
+        _apply(seq, lst)
+    grp = F0.copy()
+    j = 0
+    while j < len(lists):
+        k = j + 1
+        while k < len(lists) and flags[k]:
+            k += 1
+        snap = grp.reshape(-1).copy()
+        vals = [lst[2] * snap[lst[1]] + lst[3] for lst in lists[j:k]]
+        for lst, v in zip(lists[j:k], vals):
+            grp.reshape(-1)[lst[0]] = v
+        j = k
+    assert np.array_equal(seq, grp)
+    if name in ("karman", "vectorial"):
+        assert sum(flags) > 0, flags
+
+
+@pytest.mark.parametrize("name", ["spheres", "lid", "d1q3_walls", "karman", "vectorial", "neumann_periodic"])
+def test_resolve_step_equals_sequential_lists(name):
+    """The step's lists (ghost copy list first) applied one after another to a random
+    FP64 array leave, in every slot they store, exactly sign * f0[origin] + rhs
+    from resolve_step, for every chain that applied at most one rhs."""
+    from oracle.lbm import Simulation as O
+    from pylbm_amd.boundary import host_lists, resolve_step, unwritten_ghost_reads
+    d = {"spheres": lambda: cases.sphere_packing_d3q19(20, 14, 12, seed=3, solid_fraction=0.25),
+         "lid": lambda: cases.lid_cavity_d2q9(12),
+         "d1q3_walls": lambda: cases.d1q3_advection(10, walls=True),
+         "karman": lambda: cases.karman_d2q9(40, 20, radius=4.0, center=(10.0, 10.0)),
+         "vectorial": lambda: cases.d2q5q9_vectorial(12, walls=True),
+         "neumann_periodic": _neumann_periodic}[name]()
+    o = O(d)
+    dom, pm = _product(d)
+    for m in pm:
+        m.fix_iload()
+    lay = _HostLayout(o.ns, o.F.shape[1:])
+    lists = []
+    st, ld = unwritten_ghost_reads(dom, dom.stencil, pm, lay)
+    if st.size:
+        lists.append(host_lists("ghost_copy", lay, None, flat=(st, ld)))
+    rng = np.random.default_rng(2)
+    for m in pm:
+        m.rhs = rng.normal(size=m.rhs.shape)
+        lists.append(host_lists(m, lay, None))
+    F0 = rng.normal(size=o.F.shape)
+    seq = F0.copy()
+    for lst in lists:
+        _apply(seq, lst)
+    slot, org, sg, rh, has, bad, _, _ = resolve_step([(x[0], x[1], x[2], x[3] if np.any(x[3] != 0) else None)
+                                                for x in lists])
+    flat0, flat1 = F0.reshape(-1), seq.reshape(-1)
+    val = sg * flat0[org] + np.where(has, rh, 0.0)
+    assert slot.size > 0
+    assert np.array_equal(val[~bad], flat1[slot][~bad])
+    # slots no list stores are untouched
+    untouched = np.ones(flat0.size, bool)
+    untouched[slot] = False
+    assert np.array_equal(flat0[untouched], flat1[untouched])
+
+
+_BZ = {
+    "gap-bb": lambda: cases.poiseuille_d2q9(ny=5, y_low=1.3, y_high=2.4, method=bc.BouzidiBounceBack),
+    # loads of slots stored later in the loop: the old values, as two-phase gives
+    "gap-abb": lambda: cases.poiseuille_d2q9(ny=5, y_low=1.3, y_high=2.4, method=bc.BouzidiAntiBounceBack),
+    "karman-bz": lambda: cases.with_bouzidi(cases.karman_d2q9(48, 24, radius=4.0, center=(12.0, 12.0))),
+    "spheres-bz": lambda: cases.with_bouzidi(cases.sphere_packing_d3q19(16, 12, 12, seed=3)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(_BZ))
+def test_bouzidi_lists_equal_sequential_loop(name):
+    """The device form of an interpolated list (boundary.bouzidi_lists: last store
+    wins, gather before scatter when loads hit stores, sorted otherwise), applied
+    to a flat array of the device layout, equals the oracle's loop in link order
+    (the copy of f for bounce-back, f itself for anti-bounce-back), bit for bit."""
+    from oracle.boundary import _apply_bouzidi
+    from pylbm_amd.boundary import _flat, bouzidi_lists
+    from pylbm_amd.runtime import LBM_F64
+    from pylbm_amd.storage import layout_elements, make_layout
+    d = _BZ[name]()
+    dom, pm = _product(d)
+    _, om = _oracle(d)
+    ns = int(dom.stencil.nv_ptr[-1])
+    shape = tuple(dom.shape_halo)
+    L = make_layout(ns, shape, list(dom.stencil.vmax), LBM_F64)
+    rng = np.random.default_rng(11)
+    F = rng.standard_normal((ns,) + shape)
+    every = np.array(np.unravel_index(np.arange(F.size), F.shape))
+    pos = _flat(every, L)
+    flat = np.full(layout_elements(L), np.nan)
+    flat[pos] = F.ravel()
+    checked = 0
+    for p, o in zip(pm, om):
+        if not p.kind.bouzidi:
+            continue
+        assert o.kind == p.name
+        rhs = rng.standard_normal(p.ncond)
+        p.rhs, o.rhs = rhs.copy(), rhs.copy()
+        st, l0, l1, ca, cb, r, two, index, shift = bouzidi_lists(p, L, np.zeros(ns))
+        vals = ca * flat[l0] + cb * flat[l1] + r      # every load before any store
+        flat[st] = vals
+        _apply_bouzidi(F, o)
+        assert np.array_equal(flat[pos], F.ravel())
+        if name.startswith("gap"):
+            assert two
+        checked += 1
+    assert checked
+
+
 def test_bouzidi_anti_bounce_back_refuses_earlier_store_loads():
     """Bouzidi anti-bounce-back reads f in loop order: in a one-row gap whose upper
     wall is the nearer one, the upper links (populations pointing down, later in the
