@@ -193,10 +193,12 @@ def _moment_err(sim, ref, syms):
                for x, b in zip(syms, mref))
 
 
-def _configs_leg():
+def _configs_leg(only=None, steps_override=None):
     """The other workloads at their stated run lengths: MLUPS through Simulation.run, each
     FP32 run's moment error against the same run in FP64 on the GPU, each link list
-    timed alone, and mass drifts where the boundaries conserve mass."""
+    timed alone, and mass drifts where the boundaries conserve mass.  only: run just the
+    configurations whose name contains it; steps_override: that many steps instead of
+    the stated run lengths (both for a profiler)."""
     from pylbm_amd import Simulation, cases
 
     res = {}
@@ -211,6 +213,9 @@ def _configs_leg():
             ("C5 decaying flow 512^2 D2Q9 MRT (plain lattice run)", lambda: cases.decaying_d2q9(512),
              ("float32", "float64"), 2000, r2, True)]
     for name, make, dtypes, steps, syms, mass in plan:
+        if only and only not in name:
+            continue
+        steps = steps_override or steps
         entry = {"steps": steps}
         sims = {}
         for dt in dtypes:
@@ -230,6 +235,7 @@ def _configs_leg():
                 inner = tuple(slice(h, -h) if h else slice(None) for h in sim.domain.stencil.vmax)
                 entry["solid_fraction"] = round(1.0 - float(np.mean(sim.domain.fluid[inner])), 5)
             sims[dt] = sim
+            print("bench.py: %s %s: %.4f ms per step" % (name, dt, ms / steps), file=sys.stderr, flush=True)
         for dt in dtypes:
             if dt != "float64" and "float64" in sims:
                 entry[dt]["moment_err_vs_float64"] = float("%.3e" % _moment_err(sims[dt], sims["float64"], syms))
@@ -388,6 +394,9 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the C1/C2/C4 timings")
+    ap.add_argument("--config", default=None, help="time only the configurations whose name contains this")
+    ap.add_argument("--config-steps", type=int, default=None,
+                    help="steps per configuration run instead of the stated lengths (profiling)")
     ap.add_argument("--sur-traj", type=int, default=200, help="surrogate training trajectories")
     ap.add_argument("--sur-steps", type=int, default=2000, help="lattice steps per trajectory")
     ap.add_argument("--sur-train", type=int, default=100, help="optimizer steps per trajectory")
@@ -420,6 +429,7 @@ def main():
     alg_bytes = 2 * 19 * BYTES["float32"] * cells          # per launch
     achieved = alg_bytes * args.steps / sec / 1e9           # GB/s
     traffic = _traffic()
+    print("bench.py: %d^3 FP32: %.4f ms per step" % (n, ms / args.steps), file=sys.stderr, flush=True)
 
     extra = {}
     if not args.no_variants:
@@ -470,7 +480,7 @@ def main():
 
     del sim
     if not args.no_configs:
-        extra["configs"] = _configs_leg()
+        extra["configs"] = _configs_leg(args.config, args.config_steps)
     if not args.no_surrogate:
         extra["surrogate"] = _surrogate_leg(args.sur_traj, args.sur_steps, args.sur_train, elem=args.sur_elem)
 

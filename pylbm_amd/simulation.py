@@ -60,6 +60,15 @@ def _canon_velocities(spec, box_label=None):
     return vel
 
 
+def _no_extra_arguments(name, kwargs):
+    """pylbm's routines take keyword arguments that override the arguments of its
+    generated functions (pylbm/algorithm/base.py:669-681); the compiled routines
+    here take none, so passing one is an error instead of being ignored."""
+    if kwargs:
+        raise NotImplementedError("%s: the hip generator's routines take no extra arguments (got %s)"
+                                  % (name, ", ".join(sorted(kwargs))))
+
+
 class _MomentView:
     """Moments of boundary points handed to user value functions (m[RHO] = ...)."""
 
@@ -234,37 +243,43 @@ class Simulation:
                                             ctypes.c_void_p(m.ptr), ctypes.byref(m.layout),
                                             ctypes.byref(self.params), self.rt.stream), "f2m")
 
-    def equilibrium(self, m_user=None):
+    def equilibrium(self, m_user=None, **kwargs):
+        _no_extra_arguments("equilibrium", kwargs)
         self._equilibrium(m_user if m_user is not None else self.container.m)
 
-    def m2f(self, m_user=None, f_user=None):
+    def m2f(self, m_user=None, f_user=None, **kwargs):
+        _no_extra_arguments("m2f", kwargs)
         self._m2f(m_user if m_user is not None else self.container.m,
                   f_user if f_user is not None else self.container.F)
 
-    def f2m(self):
+    def f2m(self, **kwargs):
+        _no_extra_arguments("f2m", kwargs)
         self._f2m(self.container.F, self.container.m)
 
-    def transport(self):
+    def transport(self, **kwargs):
         """pylbm's transport phase (pylbm/simulation.py:322-327, the generated routine of
         pylbm/algorithm/base.py:273-296): Fnew[k, x] = F[k, x - v_k] on the interior.
         F's ghost slots are read as they are, and F and Fnew are not exchanged."""
+        _no_extra_arguments("transport", kwargs)
         c = self.container
         self._check(self.module.lbm_gen_transport(ctypes.c_void_p(c.F.ptr), ctypes.c_void_p(c.Fnew.ptr),
                                                   ctypes.byref(c.F.layout), ctypes.byref(self.params),
                                                   self.rt.stream), "transport")
 
-    def relaxation(self):
+    def relaxation(self, **kwargs):
         """pylbm's relaxation phase on the moment array m (pylbm/simulation.py:329-334,
         pylbm/algorithm/base.py:404-437): m_i <- (1 - s_i) m_i + s_i eq_i on every cell."""
+        _no_extra_arguments("relaxation", kwargs)
         m = self.container.m
         self._check(self.module.lbm_gen_relaxation(ctypes.c_void_p(m.ptr), ctypes.byref(m.layout),
                                                    ctypes.byref(self.params), self.rt.stream), "relaxation")
 
-    def source_term(self, fraction_of_time_step=1.0):
+    def source_term(self, fraction_of_time_step=1.0, **kwargs):
         """pylbm's source-term phase on the moment array m (pylbm/simulation.py:336-342,
         pylbm/algorithm/base.py:439-504 with ode.py:11-16): one explicit Euler half step
         m_c <- m_c + dt/2 S(m, X, Y, Z, t) on the interior at the current t.  As in
         pylbm, fraction_of_time_step is accepted and does not change the step."""
+        _no_extra_arguments("source_term", kwargs)
         m = self.container.m
         self.params.t = self.t
         self._check(self.module.lbm_gen_source_term(ctypes.c_void_p(m.ptr), ctypes.byref(m.layout),
@@ -652,7 +667,8 @@ class Simulation:
             dev.update_rhs(method.rhs, self.wshift)
 
     # ---------------------------------------------------------- stepping
-    def boundary_condition(self):
+    def boundary_condition(self, **kwargs):
+        _no_extra_arguments("boundary_condition", kwargs)
         if self._wrap:
             return   # every list folded into the step, which reads through the wrap
         f = self.container.F
@@ -675,7 +691,8 @@ class Simulation:
                                              ctypes.byref(F.layout), ctypes.byref(self.params),
                                              1 if self._wrap else 0, self.rt.stream), "one_time_step")
 
-    def one_time_step(self):
+    def one_time_step(self, **kwargs):
+        _no_extra_arguments("one_time_step", kwargs)
         if self._need_init:
             self._initialize()
         t0 = time.perf_counter() if Monitor.enabled else 0.0

@@ -780,3 +780,18 @@ def test_split_routines_match_oracle(hip, name):
     assert rel_err(got, m, floor=1e-3) <= TOL64
     # the forced box has sources in X, Y and t: the routine changed the moments
     assert np.array_equal(got, relaxed) == (name != "forced")
+
+
+def test_extra_keyword_arguments_refused(hip):
+    """pylbm's routines forward keyword arguments to its generated functions
+    (pylbm/algorithm/base.py:669-681); the compiled routines here take none, so a
+    keyword argument raises instead of being dropped, and nothing is stepped."""
+    sim = Simulation(cases.lid_cavity_d2q9(8))
+    for call in (lambda: sim.one_time_step(t=0.5), lambda: sim.boundary_condition(f=None),
+                 lambda: sim.transport(dt=1.0), lambda: sim.relaxation(m=None), lambda: sim.f2m(x=1),
+                 lambda: sim.m2f(y=2), lambda: sim.equilibrium(z=3), lambda: sim.source_term(0.5, w=4)):
+        with pytest.raises(NotImplementedError):
+            call()
+    assert sim.nt == 0
+    sim.one_time_step()
+    assert sim.nt == 1
